@@ -1,0 +1,170 @@
+/*
+ * C ABI of the exact w-gridder (vis2dirty / dirty2vis), MI355X build.
+ *
+ * The rang_core_wgridder_* symbols are the drop-in replacement of rang-core's
+ * core/src/ffi_wgridder.rs: identical names, argument order, struct layout
+ * and status codes (0 ok, 1 invalid arguments with the message kept per
+ * thread for rang_core_wgridder_last_error, 2 a failure inside the library).
+ * All buffers are host memory owned by the caller and borrowed for the call;
+ * the work runs on the GPU and outputs are complete on return.
+ *
+ * The rang_amd_* symbols have no counterpart in the reference: the same
+ * transforms on device-resident buffers and a caller's HIP stream, for
+ * callers that keep visibilities in HBM (solvers, benchmarks).
+ *
+ * Conventions (core/src/wgridder.rs:30-36): pixel (i, j) of an nx * ny image
+ * is at i * ny + j, l = lshift + (i - nx/2) dx, m = mshift + (j - ny/2) dy;
+ * vis2dirty = sum_k wgt_k Re[V_k exp(+2 pi i (u l + v m + w (n - 1)))],
+ * dirty2vis its adjoint V_k = wgt_k sum I exp(-2 pi i (...)); uvw in metres
+ * (nrow * 3 doubles), freq in Hz ascending (nchan doubles), visibilities
+ * nrow * nchan interleaved complex values, row-major.
+ */
+#ifndef RANG_CORE_WGRIDDER_H
+#define RANG_CORE_WGRIDDER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* core/src/ffi_wgridder.rs:22-43 (#[repr(C)] WgridderOptions);
+ * kernel_index < 0 lets the cost model choose kernel and grid. */
+typedef struct WgridderOptions {
+    size_t nx;
+    size_t ny;
+    double pixsize_x;
+    double pixsize_y;
+    double epsilon;
+    double center_x;
+    double center_y;
+    double sigma_min;
+    double sigma_max;
+    int32_t do_wgridding;
+    int32_t divide_by_n;
+    int32_t flip_u;
+    int32_t flip_v;
+    int32_t flip_w;
+    int32_t allow_nshift;
+    int64_t kernel_index;
+    size_t nu;
+    size_t nv;
+} WgridderOptions;
+
+/* Number of doubles written to `info` (core/src/wgridder.rs:490): kernel
+ * index, support, nu, nv, nplanes, nvis, wmin_d, wmax_d, nm1min, nm1max,
+ * nshift, dw, wmin, ofactor. */
+#define RANG_WGRIDDER_N_INFO 14
+
+/* vis_kind / wgt_kind (core/src/ffi_wgridder.rs:296-310) */
+#define WGRIDDER_VIS_UNIT 0
+#define WGRIDDER_VIS_C64 1
+#define WGRIDDER_VIS_C128 2
+#define WGRIDDER_WGT_NONE 0
+#define WGRIDDER_WGT_F32 1
+#define WGRIDDER_WGT_F64 2
+#define WGRIDDER_WGT_F32_PRODUCT 3
+
+/* core/src/ffi_wgridder.rs:155 */
+int32_t rang_core_wgridder_vis2dirty_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                         const double *uvw, const double *freq, const double *vis,
+                                         const double *wgt, const uint8_t *mask, double *dirty,
+                                         double *info);
+/* core/src/ffi_wgridder.rs:191 */
+int32_t rang_core_wgridder_vis2dirty_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                         const double *uvw, const double *freq, const float *vis,
+                                         const float *wgt, const uint8_t *mask, float *dirty,
+                                         double *info);
+/* core/src/ffi_wgridder.rs:228 */
+int32_t rang_core_wgridder_dirty2vis_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                         const double *uvw, const double *freq, const double *dirty,
+                                         const double *wgt, const uint8_t *mask, double *vis,
+                                         double *info);
+/* core/src/ffi_wgridder.rs:264 */
+int32_t rang_core_wgridder_dirty2vis_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                         const double *uvw, const double *freq, const float *dirty,
+                                         const float *wgt, const uint8_t *mask, float *vis,
+                                         double *info);
+/* core/src/ffi_wgridder.rs:454 */
+int32_t rang_core_wgridder_vis2dirty_ex_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                            const double *uvw, const double *freq, const void *vis,
+                                            int32_t vis_kind, const void *wgt, int32_t wgt_kind,
+                                            const uint8_t *mask, double *dirty, double *psf,
+                                            double *info);
+/* core/src/ffi_wgridder.rs:496 */
+int32_t rang_core_wgridder_vis2dirty_ex_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                            const double *uvw, const double *freq, const void *vis,
+                                            int32_t vis_kind, const void *wgt, int32_t wgt_kind,
+                                            const uint8_t *mask, float *dirty, float *psf,
+                                            double *info);
+/* core/src/ffi_wgridder.rs:542 */
+int32_t rang_core_wgridder_vis2dirty_ex2_f64(const WgridderOptions *opts, size_t nrow,
+                                             size_t nchan, const double *uvw, const double *freq,
+                                             const void *vis, int32_t vis_kind, const void *wgt,
+                                             const void *wgt2, int32_t wgt_kind,
+                                             const uint8_t *mask, double *dirty, double *psf,
+                                             double *info);
+/* core/src/ffi_wgridder.rs:573 */
+int32_t rang_core_wgridder_vis2dirty_ex2_f32(const WgridderOptions *opts, size_t nrow,
+                                             size_t nchan, const double *uvw, const double *freq,
+                                             const void *vis, int32_t vis_kind, const void *wgt,
+                                             const void *wgt2, int32_t wgt_kind,
+                                             const uint8_t *mask, float *dirty, float *psf,
+                                             double *info);
+/* core/src/ffi_wgridder.rs:668 */
+int32_t rang_core_wgridder_dirty2vis_ex_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                            const double *uvw, const double *freq,
+                                            const double *dirty, const void *wgt, int32_t wgt_kind,
+                                            const uint8_t *mask, double *vis, double *info);
+/* core/src/ffi_wgridder.rs:695 */
+int32_t rang_core_wgridder_dirty2vis_ex_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                            const double *uvw, const double *freq,
+                                            const float *dirty, const void *wgt, int32_t wgt_kind,
+                                            const uint8_t *mask, float *vis, double *info);
+/* core/src/ffi_wgridder.rs:722: seven doubles (W, ofactor, epsilon, beta,
+ * e0, ndim, singleprec); 1 past the end of the table. */
+int32_t rang_core_wgridder_kernel_row(size_t index, double *out);
+/* core/src/ffi_wgridder.rs:749: this thread's last status-1 message into
+ * buf (at most len bytes, not terminated); returns its full length. */
+size_t rang_core_wgridder_last_error(uint8_t *buf, size_t len);
+
+/* --- No counterpart in the reference: device-resident buffers. ----------
+ * As the _ex2 / dirty2vis_ex entry points, with every array pointer a
+ * device pointer of the current HIP device, run on `stream` (a
+ * hipStream_t, NULL for the null stream); `info` is host memory. The call
+ * synchronises `stream` before returning. */
+int32_t rang_amd_wgridder_vis2dirty_dev_f64(const WgridderOptions *opts, size_t nrow,
+                                            size_t nchan, const double *uvw, const double *freq,
+                                            const void *vis, int32_t vis_kind, const void *wgt,
+                                            const void *wgt2, int32_t wgt_kind,
+                                            const uint8_t *mask, double *dirty, double *psf,
+                                            double *info, void *stream);
+int32_t rang_amd_wgridder_vis2dirty_dev_f32(const WgridderOptions *opts, size_t nrow,
+                                            size_t nchan, const double *uvw, const double *freq,
+                                            const void *vis, int32_t vis_kind, const void *wgt,
+                                            const void *wgt2, int32_t wgt_kind,
+                                            const uint8_t *mask, float *dirty, float *psf,
+                                            double *info, void *stream);
+int32_t rang_amd_wgridder_dirty2vis_dev_f64(const WgridderOptions *opts, size_t nrow,
+                                            size_t nchan, const double *uvw, const double *freq,
+                                            const double *dirty, const void *wgt,
+                                            int32_t wgt_kind, const uint8_t *mask, double *vis,
+                                            double *info, void *stream);
+int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions *opts, size_t nrow,
+                                            size_t nchan, const double *uvw, const double *freq,
+                                            const float *dirty, const void *wgt, int32_t wgt_kind,
+                                            const uint8_t *mask, float *vis, double *info,
+                                            void *stream);
+/* Per-stage timing (HIP events on the call's stream) of this thread's later
+ * calls: on != 0 enables it; count, names and milliseconds of the last call. */
+void rang_amd_stage_timing(int32_t on);
+int32_t rang_amd_stage_count(void);
+const char *rang_amd_stage_name(int32_t i);
+double rang_amd_stage_ms(int32_t i);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,458 @@
+// The C ABI of include/rang_core_wgridder.h: rang-core's ffi_wgridder.rs
+// entry points on host buffers (staged to the GPU and back), and the
+// device-resident rang_amd_* variants.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/rang_core_wgridder.h"
+#include "wg_device.h"
+#include "wg_plan.h"
+
+using namespace rang;
+
+namespace {
+
+thread_local std::string t_last_error;
+
+int32_t fail(int32_t status, const std::string& msg) {
+    t_last_error = msg;
+    return status;
+}
+
+Options to_options(const WgridderOptions& o) {
+    Options r;
+    r.nx = o.nx;
+    r.ny = o.ny;
+    r.pixsize_x = o.pixsize_x;
+    r.pixsize_y = o.pixsize_y;
+    r.epsilon = o.epsilon;
+    r.center_x = o.center_x;
+    r.center_y = o.center_y;
+    r.sigma_min = o.sigma_min;
+    r.sigma_max = o.sigma_max;
+    r.do_wgridding = o.do_wgridding != 0;
+    r.divide_by_n = o.divide_by_n != 0;
+    r.flip_u = o.flip_u != 0;
+    r.flip_v = o.flip_v != 0;
+    r.flip_w = o.flip_w != 0;
+    r.allow_nshift = o.allow_nshift != 0;
+    r.kernel_index = o.kernel_index;
+    r.nu = o.kernel_index >= 0 ? o.nu : 0;
+    r.nv = o.kernel_index >= 0 ? o.nv : 0;
+    return r;
+}
+
+std::size_t vis_bytes(int kind) { return kind == kVisC64 ? 8 : kind == kVisC128 ? 16 : 0; }
+std::size_t wgt_bytes(int kind) {
+    return kind == kWgtF32 || kind == kWgtF32Product ? 4 : kind == kWgtF64 ? 8 : 0;
+}
+
+// Checks of the kinds and pointers (core/src/ffi_wgridder.rs:318-370).
+bool kinds_ok(const void* vis, int vis_kind, const void* wgt, const void* wgt2, int wgt_kind,
+              std::string& err) {
+    if (!(vis_kind == kVisUnit || ((vis_kind == kVisC64 || vis_kind == kVisC128) && vis))) {
+        err = "invalid visibility array or kind";
+        return false;
+    }
+    const bool wok = wgt_kind == kWgtNone || ((wgt_kind == kWgtF32 || wgt_kind == kWgtF64) && wgt) ||
+                     (wgt_kind == kWgtF32Product && wgt && wgt2);
+    if (!wok) {
+        err = "invalid weight array or kind";
+        return false;
+    }
+    return true;
+}
+
+// The checks that need no GPU, in the reference's order
+// (core/src/wgridder.rs:2449-2462): options, then the frequencies.
+bool host_checks(const Options& o, std::size_t nrow, std::size_t nchan, const double* freq,
+                 std::string& err) {
+    if (!check_common(o, nrow, nchan, err)) return false;
+    if (nrow * nchan >= (std::size_t(1) << 31)) {
+        err = "too many visibilities for one call (at most 2^31 - 1)";
+        return false;
+    }
+    return check_freq(freq, nchan, err);
+}
+
+// Host staging buffers of one device and the stream the host-buffer entry
+// points run on.
+struct Staging {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    void* ptr[8] = {};
+    std::size_t size[8] = {};
+    void* get(int slot, std::size_t bytes) {
+        if (bytes == 0) return nullptr;
+        if (size[slot] < bytes) {
+            if (ptr[slot]) (void)hipFree(ptr[slot]);
+            ptr[slot] = nullptr;
+            size[slot] = 0;
+            if (hipMalloc(&ptr[slot], bytes) != hipSuccess) return nullptr;
+            size[slot] = bytes;
+        }
+        return ptr[slot];
+    }
+};
+
+Staging* staging() {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Staging>> all;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& s = all[dev];
+    if (!s) {
+        s.reset(new Staging());
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+            s.reset();
+            return nullptr;
+        }
+    }
+    return s.get();
+}
+
+struct Stage {
+    Staging* s;
+    std::string err;
+    bool ok = true;
+    template <class P>
+    P* up(int slot, const P* host, std::size_t bytes) {
+        if (!host || bytes == 0 || !ok) return nullptr;
+        void* d = s->get(slot, bytes);
+        if (!d || hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+            ok = false;
+            err = "staging to the GPU failed";
+            return nullptr;
+        }
+        return static_cast<P*>(d);
+    }
+    void* out(int slot, const void* host, std::size_t bytes) {
+        if (!host || bytes == 0 || !ok) return nullptr;
+        void* d = s->get(slot, bytes);
+        if (!d) {
+            ok = false;
+            err = "device allocation failed";
+        }
+        return d;
+    }
+    bool down(void* host, const void* dev, std::size_t bytes) {
+        if (!host || !dev || bytes == 0) return true;
+        if (hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s->stream) != hipSuccess) return false;
+        return true;
+    }
+};
+
+template <class T>
+int32_t v2d_host(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
+                 const double* uvw, const double* freq, const void* vis, int32_t vis_kind,
+                 const void* wgt, const void* wgt2, int32_t wgt_kind, const std::uint8_t* mask,
+                 T* dirty, T* psf, double* info) {
+    try {
+        if (!opts) return fail(1, "null options");
+        if (!uvw || !freq || !info || (!dirty && !psf)) return fail(1, "null array");
+        std::string err;
+        if (!kinds_ok(vis, vis_kind, wgt, wgt2, wgt_kind, err)) return fail(1, err);
+        const Options o = to_options(*opts);
+        if (!host_checks(o, nrow, nchan, freq, err)) return fail(1, err);
+        Staging* s = staging();
+        if (!s) return fail(2, "no HIP device");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const std::size_t n = nrow * nchan, npix = o.nx * o.ny;
+        Stage st{s};
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = st.up(0, uvw, sizeof(double) * 3 * nrow);
+        in.freq = st.up(1, freq, sizeof(double) * nchan);
+        in.vis = st.up(2, static_cast<const char*>(vis), vis_bytes(vis_kind) * n);
+        in.vis_kind = vis_kind;
+        in.wgt = st.up(3, static_cast<const char*>(wgt), wgt_bytes(wgt_kind) * n);
+        in.wgt2 = wgt_kind == kWgtF32Product ? st.up(4, static_cast<const char*>(wgt2), 4 * n) : nullptr;
+        in.wgt_kind = wgt_kind;
+        in.mask = st.up(5, mask, n);
+        T* ddirty = static_cast<T*>(st.out(6, dirty, sizeof(T) * npix));
+        T* dpsf = static_cast<T*>(st.out(7, psf, sizeof(T) * npix));
+        if (!st.ok) return fail(2, st.err);
+        const int32_t status = device_vis2dirty<T>(o, in, freq, ddirty, dpsf, info, s->stream, err);
+        if (status != 0) return fail(status, err);
+        if (!st.down(dirty, ddirty, sizeof(T) * npix) || !st.down(psf, dpsf, sizeof(T) * npix) ||
+            hipStreamSynchronize(s->stream) != hipSuccess)
+            return fail(2, "copy from the GPU failed");
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+template <class T>
+int32_t d2v_host(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
+                 const double* uvw, const double* freq, const T* dirty, const void* wgt,
+                 int32_t wgt_kind, const std::uint8_t* mask, T* vis, double* info) {
+    try {
+        if (!opts) return fail(1, "null options");
+        if (!uvw || !freq || !vis || !dirty || !info) return fail(1, "null array");
+        std::string err;
+        if (wgt_kind == kWgtF32Product || !kinds_ok(nullptr, kVisUnit, wgt, nullptr, wgt_kind, err))
+            return fail(1, err.empty() ? "invalid weight array or kind" : err);
+        const Options o = to_options(*opts);
+        if (!host_checks(o, nrow, nchan, freq, err)) return fail(1, err);
+        Staging* s = staging();
+        if (!s) return fail(2, "no HIP device");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const std::size_t n = nrow * nchan, npix = o.nx * o.ny;
+        Stage st{s};
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = st.up(0, uvw, sizeof(double) * 3 * nrow);
+        in.freq = st.up(1, freq, sizeof(double) * nchan);
+        in.wgt = st.up(3, static_cast<const char*>(wgt), wgt_bytes(wgt_kind) * n);
+        in.wgt_kind = wgt_kind;
+        in.mask = st.up(5, mask, n);
+        const T* ddirty = st.up(6, dirty, sizeof(T) * npix);
+        T* dvis = static_cast<T*>(st.out(7, vis, sizeof(T) * 2 * n));
+        if (!st.ok) return fail(2, st.err);
+        if (!dvis && n > 0) return fail(2, "device allocation failed");
+        if (n == 0) {
+            // nothing to degrid; the reference still checks the options
+            std::string e2;
+            if (!check_common(o, nrow, nchan, e2)) return fail(1, e2);
+            if (!check_freq(freq, nchan, e2)) return fail(1, e2);
+            std::fill(info, info + kNInfo, 0.0);
+            return 0;
+        }
+        const int32_t status = device_dirty2vis<T>(o, in, freq, ddirty, dvis, info, s->stream, err);
+        if (status != 0) return fail(status, err);
+        if (!st.down(vis, dvis, sizeof(T) * 2 * n) || hipStreamSynchronize(s->stream) != hipSuccess)
+            return fail(2, "copy from the GPU failed");
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+bool freq_to_host(const double* freq_dev, std::size_t nchan, std::vector<double>& out,
+                  std::string& err) {
+    out.assign(nchan, 0.0);
+    if (nchan && hipMemcpy(out.data(), freq_dev, sizeof(double) * nchan, hipMemcpyDeviceToHost) != hipSuccess) {
+        err = "cannot read the channel frequencies from the device";
+        return false;
+    }
+    return true;
+}
+
+template <class T>
+int32_t v2d_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
+                const double* uvw, const double* freq, const void* vis, int32_t vis_kind,
+                const void* wgt, const void* wgt2, int32_t wgt_kind, const std::uint8_t* mask,
+                T* dirty, T* psf, double* info, void* stream) {
+    try {
+        if (!opts) return fail(1, "null options");
+        if (!uvw || !freq || !info || (!dirty && !psf)) return fail(1, "null array");
+        std::string err;
+        if (!kinds_ok(vis, vis_kind, wgt, wgt2, wgt_kind, err)) return fail(1, err);
+        std::vector<double> fh;
+        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err);
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = uvw;
+        in.freq = freq;
+        in.vis = vis;
+        in.vis_kind = vis_kind;
+        in.wgt = wgt;
+        in.wgt2 = wgt2;
+        in.wgt_kind = wgt_kind;
+        in.mask = mask;
+        const int32_t status = device_vis2dirty<T>(to_options(*opts), in, fh.data(), dirty, psf, info,
+                                                   static_cast<hipStream_t>(stream), err);
+        return status ? fail(status, err) : 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+template <class T>
+int32_t d2v_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
+                const double* uvw, const double* freq, const T* dirty, const void* wgt,
+                int32_t wgt_kind, const std::uint8_t* mask, T* vis, double* info, void* stream) {
+    try {
+        if (!opts) return fail(1, "null options");
+        if (!uvw || !freq || !vis || !dirty || !info) return fail(1, "null array");
+        std::string err;
+        if (wgt_kind == kWgtF32Product || !kinds_ok(nullptr, kVisUnit, wgt, nullptr, wgt_kind, err))
+            return fail(1, err.empty() ? "invalid weight array or kind" : err);
+        std::vector<double> fh;
+        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err);
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = uvw;
+        in.freq = freq;
+        in.wgt = wgt;
+        in.wgt_kind = wgt_kind;
+        in.mask = mask;
+        const int32_t status = device_dirty2vis<T>(to_options(*opts), in, fh.data(), dirty, vis, info,
+                                                   static_cast<hipStream_t>(stream), err);
+        return status ? fail(status, err) : 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rang_core_wgridder_vis2dirty_f64(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                         const double* uvw, const double* freq, const double* vis,
+                                         const double* wgt, const uint8_t* mask, double* dirty,
+                                         double* info) {
+    if (!vis || !dirty) return fail(1, "null array");
+    return v2d_host<double>(opts, nrow, nchan, uvw, freq, vis, kVisC128, wgt, nullptr,
+                            wgt ? kWgtF64 : kWgtNone, mask, dirty, nullptr, info);
+}
+
+int32_t rang_core_wgridder_vis2dirty_f32(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                         const double* uvw, const double* freq, const float* vis,
+                                         const float* wgt, const uint8_t* mask, float* dirty,
+                                         double* info) {
+    if (!vis || !dirty) return fail(1, "null array");
+    return v2d_host<float>(opts, nrow, nchan, uvw, freq, vis, kVisC64, wgt, nullptr,
+                           wgt ? kWgtF32 : kWgtNone, mask, dirty, nullptr, info);
+}
+
+int32_t rang_core_wgridder_dirty2vis_f64(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                         const double* uvw, const double* freq, const double* dirty,
+                                         const double* wgt, const uint8_t* mask, double* vis,
+                                         double* info) {
+    return d2v_host<double>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt ? kWgtF64 : kWgtNone, mask,
+                            vis, info);
+}
+
+int32_t rang_core_wgridder_dirty2vis_f32(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                         const double* uvw, const double* freq, const float* dirty,
+                                         const float* wgt, const uint8_t* mask, float* vis,
+                                         double* info) {
+    return d2v_host<float>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt ? kWgtF32 : kWgtNone, mask,
+                           vis, info);
+}
+
+int32_t rang_core_wgridder_vis2dirty_ex_f64(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                            const double* uvw, const double* freq, const void* vis,
+                                            int32_t vis_kind, const void* wgt, int32_t wgt_kind,
+                                            const uint8_t* mask, double* dirty, double* psf,
+                                            double* info) {
+    return v2d_host<double>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, nullptr, wgt_kind,
+                            mask, dirty, psf, info);
+}
+
+int32_t rang_core_wgridder_vis2dirty_ex_f32(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                            const double* uvw, const double* freq, const void* vis,
+                                            int32_t vis_kind, const void* wgt, int32_t wgt_kind,
+                                            const uint8_t* mask, float* dirty, float* psf,
+                                            double* info) {
+    return v2d_host<float>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, nullptr, wgt_kind,
+                           mask, dirty, psf, info);
+}
+
+int32_t rang_core_wgridder_vis2dirty_ex2_f64(const WgridderOptions* opts, size_t nrow,
+                                             size_t nchan, const double* uvw, const double* freq,
+                                             const void* vis, int32_t vis_kind, const void* wgt,
+                                             const void* wgt2, int32_t wgt_kind,
+                                             const uint8_t* mask, double* dirty, double* psf,
+                                             double* info) {
+    return v2d_host<double>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind,
+                            mask, dirty, psf, info);
+}
+
+int32_t rang_core_wgridder_vis2dirty_ex2_f32(const WgridderOptions* opts, size_t nrow,
+                                             size_t nchan, const double* uvw, const double* freq,
+                                             const void* vis, int32_t vis_kind, const void* wgt,
+                                             const void* wgt2, int32_t wgt_kind,
+                                             const uint8_t* mask, float* dirty, float* psf,
+                                             double* info) {
+    return v2d_host<float>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind, mask,
+                           dirty, psf, info);
+}
+
+int32_t rang_core_wgridder_dirty2vis_ex_f64(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                            const double* uvw, const double* freq,
+                                            const double* dirty, const void* wgt, int32_t wgt_kind,
+                                            const uint8_t* mask, double* vis, double* info) {
+    return d2v_host<double>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt_kind, mask, vis, info);
+}
+
+int32_t rang_core_wgridder_dirty2vis_ex_f32(const WgridderOptions* opts, size_t nrow, size_t nchan,
+                                            const double* uvw, const double* freq,
+                                            const float* dirty, const void* wgt, int32_t wgt_kind,
+                                            const uint8_t* mask, float* vis, double* info) {
+    return d2v_host<float>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt_kind, mask, vis, info);
+}
+
+int32_t rang_core_wgridder_kernel_row(size_t index, double* out) {
+    const KernelRow* r = kernel_row(index);
+    if (!r || !out) return 1;
+    const double v[7] = {double(r->w), r->ofactor, r->epsilon, r->beta, r->e0, double(r->ndim),
+                         double(r->single)};
+    std::memcpy(out, v, sizeof v);
+    return 0;
+}
+
+size_t rang_core_wgridder_last_error(uint8_t* buf, size_t len) {
+    if (buf) std::memcpy(buf, t_last_error.data(), std::min(len, t_last_error.size()));
+    return t_last_error.size();
+}
+
+int32_t rang_amd_wgridder_vis2dirty_dev_f64(const WgridderOptions* opts, size_t nrow,
+                                            size_t nchan, const double* uvw, const double* freq,
+                                            const void* vis, int32_t vis_kind, const void* wgt,
+                                            const void* wgt2, int32_t wgt_kind,
+                                            const uint8_t* mask, double* dirty, double* psf,
+                                            double* info, void* stream) {
+    return v2d_dev<double>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind, mask,
+                           dirty, psf, info, stream);
+}
+
+int32_t rang_amd_wgridder_vis2dirty_dev_f32(const WgridderOptions* opts, size_t nrow,
+                                            size_t nchan, const double* uvw, const double* freq,
+                                            const void* vis, int32_t vis_kind, const void* wgt,
+                                            const void* wgt2, int32_t wgt_kind,
+                                            const uint8_t* mask, float* dirty, float* psf,
+                                            double* info, void* stream) {
+    return v2d_dev<float>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind, mask,
+                          dirty, psf, info, stream);
+}
+
+int32_t rang_amd_wgridder_dirty2vis_dev_f64(const WgridderOptions* opts, size_t nrow,
+                                            size_t nchan, const double* uvw, const double* freq,
+                                            const double* dirty, const void* wgt,
+                                            int32_t wgt_kind, const uint8_t* mask, double* vis,
+                                            double* info, void* stream) {
+    return d2v_dev<double>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt_kind, mask, vis, info,
+                           stream);
+}
+
+int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions* opts, size_t nrow,
+                                            size_t nchan, const double* uvw, const double* freq,
+                                            const float* dirty, const void* wgt, int32_t wgt_kind,
+                                            const uint8_t* mask, float* vis, double* info,
+                                            void* stream) {
+    return d2v_dev<float>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt_kind, mask, vis, info,
+                          stream);
+}
+
+void rang_amd_stage_timing(int32_t on) { set_stage_timing(on != 0); }
+int32_t rang_amd_stage_count(void) { return stage_timing_count(); }
+const char* rang_amd_stage_name(int32_t i) { return stage_timing_name(i); }
+double rang_amd_stage_ms(int32_t i) { return stage_timing_ms(i); }
+
+}  // extern "C"

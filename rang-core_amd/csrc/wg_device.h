@@ -1,0 +1,49 @@
+// Device pipeline of the exact w-gridder: vis2dirty and dirty2vis on
+// device-resident inputs, on one HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "wg_plan.h"
+
+namespace rang {
+
+// Kinds of the reference's `_ex` entry points (core/src/ffi_wgridder.rs:296-310).
+enum VisKind : int { kVisUnit = 0, kVisC64 = 1, kVisC128 = 2 };
+enum WgtKind : int { kWgtNone = 0, kWgtF32 = 1, kWgtF64 = 2, kWgtF32Product = 3 };
+
+// Device pointers of one call's inputs.
+struct DevInputs {
+    std::size_t nrow = 0, nchan = 0;
+    const double* uvw = nullptr;   // nrow * 3
+    const double* freq = nullptr;  // nchan
+    const void* vis = nullptr;
+    int vis_kind = kVisUnit;
+    const void* wgt = nullptr;
+    const void* wgt2 = nullptr;
+    int wgt_kind = kWgtNone;
+    const std::uint8_t* mask = nullptr;
+};
+
+// Status as the reference's C ABI: 0 ok, 1 invalid arguments (message in
+// err), 2 failure of the device (message in err).
+template <class T>
+int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
+                     T* psf, double* info, hipStream_t stream, std::string& err);
+
+template <class T>
+int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
+                     const T* dirty, T* vis, double* info, hipStream_t stream, std::string& err);
+
+// Per-kernel timing of the last call on this thread (hipEvents on the call's
+// stream; enabled by set_stage_timing). Names and milliseconds.
+void set_stage_timing(bool on);
+int stage_timing_count();
+const char* stage_timing_name(int i);
+double stage_timing_ms(int i);
+
+}  // namespace rang

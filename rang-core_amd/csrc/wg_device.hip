@@ -1,0 +1,1413 @@
+// Device pipeline of the exact w-gridder for CDNA4 (gfx950).
+//
+// vis2dirty (rang-core core/src/wgridder.rs:2436-2683):
+//   scan      active count and |w| range                    (k_scan)
+//   plan      kernel, grid and w-planes on the host, FP64    (wg_plan.cpp)
+//   sort      bucket key per visibility, counts, prefix sum, (k_keys, scan,
+//             one record per active visibility in bucket order  k_scatter)
+//   grid      one workgroup per (uv tile, group of w-planes) gathers the
+//             visibilities of its tile and of the three tiles whose
+//             footprints reach into it, accumulates in LDS and writes its
+//             tile of each plane once: no global atomics  (k_grid)
+//   v FFT     inverse along v of every row of every plane, keeping the image
+//             columns                                     (k_rows_inv)
+//   u FFT     inverse along u of the image columns, projected with the
+//             plane's w-screen                             (k_cols_inv)
+//   sum       plane terms added in plane order, global correction (k_sum)
+// dirty2vis (wgridder.rs:2998-3163) runs the transposed chain: corrected
+// image, u FFT with the w-screen, v FFT, then degridding from the planes with
+// the weight and shift phase applied per visibility (k_degrid).
+//
+// FP64 is used for the uvw x f/c coordinates, the fractional grid position,
+// the w coordinate of each plane, the w-screen phase (reduced mod 1 in single
+// precision as the reference does) and the correction factors; gridding,
+// degridding and the FFTs run in the working precision T.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "wg_device.h"
+#include "wg_fft.h"
+
+namespace rang {
+
+namespace {
+
+struct DevError {
+    std::string msg;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw DevError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+#define HIPCHK(x) check((x), #x)
+
+// ------------------------------------------------------------ workspace
+
+enum Slot : int {
+    kSlotFoc,
+    kSlotCounters,
+    kSlotKeys,
+    kSlotCounts,
+    kSlotOffsets,
+    kSlotCursor,
+    kSlotScan,
+    kSlotRecs,
+    kSlotCoeff,
+    kSlotTwU,
+    kSlotTwV,
+    kSlotGL,
+    kSlotCfu,
+    kSlotCfv,
+    kSlotScreen,
+    kSlotCorr,
+    kSlotGrid,
+    kSlotTmp,
+    kSlotProj,
+    kSlotImg,
+    kSlotCount
+};
+
+// Device buffers of one GPU kept between calls and grown on demand; calls on
+// one device are serialised by its mutex.
+struct Workspace {
+    void* ptr[kSlotCount] = {};
+    std::size_t size[kSlotCount] = {};
+    std::mutex mu;
+    void* get(int slot, std::size_t bytes) {
+        bytes = std::max<std::size_t>(bytes, 256);
+        if (size[slot] < bytes) {
+            if (ptr[slot]) HIPCHK(hipFree(ptr[slot]));
+            ptr[slot] = nullptr;
+            size[slot] = 0;
+            HIPCHK(hipMalloc(&ptr[slot], bytes));
+            size[slot] = bytes;
+        }
+        return ptr[slot];
+    }
+};
+
+Workspace& workspace(int device) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Workspace>> all;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& w = all[device];
+    if (!w) w.reset(new Workspace());
+    return *w;
+}
+
+// ------------------------------------------------------------ stage timing
+
+thread_local bool g_timing = false;
+thread_local std::vector<std::pair<std::string, double>> g_times;
+
+struct StageTimer {
+    hipStream_t st;
+    std::vector<std::pair<std::string, hipEvent_t>> marks;
+    explicit StageTimer(hipStream_t s) : st(s) {
+        if (g_timing) mark("start");
+    }
+    void mark(const char* name) {
+        if (!g_timing) return;
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, st));
+        marks.emplace_back(name, e);
+    }
+    void finish() {
+        if (!g_timing) return;
+        g_times.clear();
+        HIPCHK(hipEventSynchronize(marks.back().second));
+        for (std::size_t i = 1; i < marks.size(); ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
+            g_times.emplace_back(marks[i].first, double(ms));
+        }
+    }
+    ~StageTimer() {
+        for (auto& m : marks) (void)hipEventDestroy(m.second);
+    }
+};
+
+// ------------------------------------------------------------ geometry
+
+// What the kernels read of the call's geometry (wg_plan.h Geometry plus the
+// tiling), by value.
+struct Geo {
+    long long n;
+    int nrow, nchan;
+    int nx, ny, nu, nv, supp;
+    double pixsize_x, pixsize_y, ushift, vshift;
+    long long maxiu0, maxiv0;
+    double wmin, dw, xdw, wshift;
+    int nplanes, nb, wgrid;
+    double r_l, r_m, lshift, mshift, nshift;
+    int shifting, lmshift;
+    double fu, fv, fw;
+    int Tu, Tv, ntu, ntv;
+    int tnx, tny;
+    int divide_by_n;
+};
+
+struct DevIn {
+    const double* uvw;
+    const double* foc;
+    const void* vis;
+    const void* wgt;
+    const void* wgt2;
+    const std::uint8_t* mask;
+    int vis_kind, wgt_kind;
+};
+
+// One active visibility in bucket order. v2d: (re, im) its stored value
+// (wgridder.rs:1439-1475); dirty2vis: the weight times the conjugate shift
+// phase. idx: row * nchan + channel, bit 31 set where FixW flipped the row.
+template <class T>
+struct Rec {
+    double w;
+    T x0, y0;
+    T re, im;
+    std::uint32_t pos;  // ou | ov << 8 | first plane << 16
+    std::uint32_t idx;
+};
+
+constexpr std::uint32_t kFlip = 0x80000000u;
+constexpr std::uint32_t kNoKey = 0xffffffffu;
+
+__device__ __forceinline__ void coord_of(const DevIn& in, const Geo& g, int row, double c[3]) {
+    const double tu = g.fu * in.uvw[3 * row], tv = g.fv * in.uvw[3 * row + 1];
+    const double tw = g.fw * in.uvw[3 * row + 2];
+    c[0] = tu + g.r_l * tw;
+    c[1] = tv + g.r_m * tw;
+    c[2] = tw;
+}
+
+__device__ __forceinline__ void sincos_t(float a, float& s, float& c) { sincosf(a, &s, &c); }
+__device__ __forceinline__ void sincos_t(double a, double& s, double& c) { sincos(a, &s, &c); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+
+template <class T>
+__device__ __forceinline__ cx<T> vis_get(const DevIn& in, long long i) {
+    if (in.vis_kind == kVisC64) {
+        const float2 f = static_cast<const float2*>(in.vis)[i];
+        return {T(double(f.x)), T(double(f.y))};
+    }
+    if (in.vis_kind == kVisC128) {
+        const double2 f = static_cast<const double2*>(in.vis)[i];
+        return {T(f.x), T(f.y)};
+    }
+    return {T(1), T(0)};
+}
+
+template <class T>
+__device__ __forceinline__ bool wgt_get(const DevIn& in, long long i, T& w) {
+    switch (in.wgt_kind) {
+    case kWgtF32:
+        w = T(double(static_cast<const float*>(in.wgt)[i]));
+        return true;
+    case kWgtF64:
+        w = T(static_cast<const double*>(in.wgt)[i]);
+        return true;
+    case kWgtF32Product: {
+        const float p = __fmul_rn(static_cast<const float*>(in.wgt)[i],
+                                  static_cast<const float*>(in.wgt2)[i]);
+        w = T(double(p));
+        return true;
+    }
+    default:
+        return false;
+    }
+}
+
+// Activity (wgridder.rs:1033-1041): |vis|^2 wgt mask != 0 for a dirty
+// image alone, wgt mask != 0 otherwise.
+template <class T, bool USE_VIS>
+__device__ __forceinline__ bool active(const DevIn& in, long long i) {
+    double tnorm = 1.0;
+    if (USE_VIS) {
+        // |v|^2 in T without contraction, as num_complex's norm_sqr
+        const cx<T> v = vis_get<T>(in, i);
+        tnorm = double(add_rn(mul_rn(v.re, v.re), mul_rn(v.im, v.im)));
+    }
+    T wv;
+    const double twgt = wgt_get<T>(in, i, wv) ? double(wv) : 1.0;
+    const double tmask = in.mask ? double(in.mask[i]) : 1.0;
+    return tnorm * twgt * tmask != 0.0;
+}
+
+__device__ __forceinline__ void getpix(const Geo& g, double uin, double vin, double& uf,
+                                       double& vf, long long& iu0, long long& iv0) {
+    double u = uin * g.pixsize_x;
+    u = (u - floor(u)) * double(g.nu);
+    iu0 = min((long long)(u + g.ushift) - g.nu, g.maxiu0);
+    uf = u - double(iu0);
+    double v = vin * g.pixsize_y;
+    v = (v - floor(v)) * double(g.nv);
+    iv0 = min((long long)(v + g.vshift) - g.nv, g.maxiv0);
+    vf = v - double(iv0);
+}
+
+__device__ __forceinline__ int shifted(long long i0, int n) {
+    long long v = (i0 + n / 2) % n;
+    return int(v < 0 ? v + n : v);
+}
+
+__device__ __forceinline__ int minplane(const Geo& g, double w) {
+    if (!g.wgrid) return 0;
+    const double p = (w + g.wshift) * g.xdw;
+    const int m = p > 0.0 ? int(min(p, 2147483647.0)) : 0;
+    return min(m, g.nb - 1);
+}
+
+// Shift phase of a visibility (wgridder.rs:1395-1408).
+template <class T>
+__device__ __forceinline__ cx<T> shift_phase(const Geo& g, const double b[3], double imflip,
+                                             double foc, double sign) {
+    const double fct = sign * imflip * (b[0] * g.lshift + b[1] * g.mshift + b[2] * g.nshift);
+    const double tmp = fct * foc;
+    T ang;
+    if constexpr (sizeof(T) == 4)
+        ang = T(2.0 * M_PI * (tmp - floor(tmp)));
+    else
+        ang = T(2.0 * M_PI * tmp);
+    T s, c;
+    sincos_t(ang, s, c);
+    return {c, s};
+}
+
+
+// ------------------------------------------------------------ scan
+
+template <class T, bool USE_VIS>
+__global__ __launch_bounds__(256) void k_scan(DevIn in, Geo g, unsigned long long* out) {
+    unsigned long long cnt = 0;
+    double lo = 1e300, hi = -1e300;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
+        if (!active<T, USE_VIS>(in, i)) continue;
+        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
+        const double w = fabs(g.fw * in.uvw[3 * row + 2] * in.foc[ch]);
+        ++cnt;
+        lo = fmin(lo, w);
+        hi = fmax(hi, w);
+    }
+    __shared__ unsigned long long s_cnt[256];
+    __shared__ double s_lo[256], s_hi[256];
+    s_cnt[threadIdx.x] = cnt;
+    s_lo[threadIdx.x] = lo;
+    s_hi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+            s_lo[threadIdx.x] = fmin(s_lo[threadIdx.x], s_lo[threadIdx.x + h]);
+            s_hi[threadIdx.x] = fmax(s_hi[threadIdx.x], s_hi[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && s_cnt[0] > 0) {
+        atomicAdd(&out[0], s_cnt[0]);
+        // |w| >= 0: the bit patterns order as the values
+        atomicMin(&out[1], (unsigned long long)__double_as_longlong(s_lo[0]));
+        atomicMax(&out[2], (unsigned long long)__double_as_longlong(s_hi[0]));
+    }
+}
+
+// ------------------------------------------------------------ sort
+
+// Bucket of an active visibility: (tile, spill class, first plane). The
+// spill class says whether its footprint reaches the next tile in u (2) and
+// in v (1).
+template <class T, bool USE_VIS>
+__global__ __launch_bounds__(256) void k_keys(DevIn in, Geo g, std::uint32_t* keys,
+                                              std::uint32_t* counts) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
+        if (!active<T, USE_VIS>(in, i)) {
+            keys[i] = kNoKey;
+            continue;
+        }
+        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
+        double c[3];
+        coord_of(in, g, row, c);
+        const double sgn = c[2] < 0.0 ? -1.0 : 1.0;
+        const double f = in.foc[ch];
+        const double u = c[0] * sgn * f, v = c[1] * sgn * f, w = c[2] * sgn * f;
+        double uf, vf;
+        long long iu0, iv0;
+        getpix(g, u, v, uf, vf, iu0, iv0);
+        const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
+        const int tu = su / g.Tu, tv = sv / g.Tv;
+        const int ou = su - tu * g.Tu, ov = sv - tv * g.Tv;
+        const int cat = ((ou + g.supp > g.Tu) ? 2 : 0) | ((ov + g.supp > g.Tv) ? 1 : 0);
+        const std::uint32_t key = std::uint32_t((((long long)tu * g.ntv + tv) * 4 + cat) * g.nb +
+                                                minplane(g, w));
+        keys[i] = key;
+        atomicAdd(&counts[key], 1u);
+    }
+}
+
+constexpr int kScanItems = 4096;  // per block of 1024 threads
+
+// Exclusive prefix sum of n values in blocks of kScanItems; block totals to
+// sums.
+__global__ __launch_bounds__(1024) void k_scan_block(const std::uint32_t* in, std::uint32_t* out,
+                                                     std::uint32_t* sums, long long n) {
+    __shared__ std::uint32_t s[1024];
+    const long long base = (long long)blockIdx.x * kScanItems + threadIdx.x * 4;
+    std::uint32_t v[4], tot = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    s[threadIdx.x] = tot;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        std::uint32_t add = threadIdx.x >= unsigned(off) ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    std::uint32_t run = s[threadIdx.x] - tot;
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(std::uint32_t* out, const std::uint32_t* offs,
+                                                  long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += offs[i / kScanItems];
+}
+
+// One record per active visibility in bucket order. MODE 0: a dirty image
+// alone (the data value), 1: with a PSF (the unit value; the data value is
+// formed while gridding), 2: dirty2vis (weight times conjugate shift phase).
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_scatter(DevIn in, Geo g, const std::uint32_t* keys,
+                                                 const std::uint32_t* offsets,
+                                                 std::uint32_t* cursor, Rec<T>* recs) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
+        const std::uint32_t key = keys[i];
+        if (key == kNoKey) continue;
+        const std::uint32_t slot = offsets[key] + atomicAdd(&cursor[key], 1u);
+        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
+        double c[3];
+        coord_of(in, g, row, c);
+        const double imflip = c[2] < 0.0 ? -1.0 : 1.0;  // FixW (wgridder.rs:583-586)
+        const double b[3] = {c[0] * imflip, c[1] * imflip, c[2] * imflip};
+        const double f = in.foc[ch];
+        const double u = b[0] * f, v = b[1] * f, w = b[2] * f;
+        double uf, vf;
+        long long iu0, iv0;
+        getpix(g, u, v, uf, vf, iu0, iv0);
+        const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
+        const int ou = su % g.Tu, ov = sv % g.Tv;
+        Rec<T> r;
+        r.w = w;
+        r.x0 = T(-uf * 2.0 + (double(g.supp) - 1.0));
+        r.y0 = T(-vf * 2.0 + (double(g.supp) - 1.0));
+        r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8) |
+                (std::uint32_t(key % std::uint32_t(g.nb)) << 16);
+        r.idx = std::uint32_t(i) | (imflip < 0.0 ? kFlip : 0u);
+        T wv = T(1);
+        const bool haswgt = wgt_get<T>(in, i, wv);
+        if constexpr (MODE == 2) {
+            // wgridder.rs:3146-3159
+            if (g.shifting) {
+                const cx<T> ph = shift_phase<T>(g, b, imflip, f, -1.0);
+                r.re = ph.re * wv;
+                r.im = ph.im * wv;
+            } else {
+                r.re = wv;
+                r.im = T(0);
+            }
+        } else {
+            // unit value (wgridder.rs:1414-1424)
+            cx<T> p = {T(1), T(0)};
+            if (g.shifting) p = cmul(p, shift_phase<T>(g, b, imflip, f, 1.0));
+            if (haswgt) p = {p.re * wv, p.im * wv};
+            p.im = p.im * T(imflip);
+            if constexpr (MODE == 0) {
+                // data value (wgridder.rs:1432-1435)
+                const cx<T> d = cmul(vis_get<T>(in, i), cx<T>{p.re, p.im * T(imflip)});
+                p = {d.re, d.im * T(imflip)};
+            }
+            r.re = p.re;
+            r.im = p.im;
+        }
+        recs[slot] = r;
+    }
+}
+
+// ------------------------------------------------------------ kernel taps
+
+template <int S>
+constexpr int ne_of() {
+    return (((S + 3) | 1) + 1) / 2;
+}
+
+// u and v taps of one visibility by Horner's rule in x^2 on the even and odd
+// parts, the second half mirrored (wgridder.rs:1514-1543).
+template <class T, int S>
+__device__ __forceinline__ void eval_taps2(const T* c, T x, T y, T* ru, T* rv) {
+    constexpr int NE = ne_of<S>(), H = (S + 1) / 2;
+    const T x2 = x * x, y2 = y * y;
+    T ex[H], ey[H], ox[H], oy[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        ex[i] = ey[i] = c[i];
+        ox[i] = oy[i] = c[NE * S + i];
+    }
+#pragma unroll
+    for (int k = 1; k < NE; ++k) {
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            ex[i] = fma(ex[i], x2, c[k * S + i]);
+            ey[i] = fma(ey[i], y2, c[k * S + i]);
+            ox[i] = fma(ox[i], x2, c[(NE + k) * S + i]);
+            oy[i] = fma(oy[i], y2, c[(NE + k) * S + i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        ru[i] = fma(ox[i], x, ex[i]);
+        rv[i] = fma(oy[i], y, ey[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < S - H; ++i) {
+        ru[S - 1 - i] = fma(ox[i], -x, ex[i]);
+        rv[S - 1 - i] = fma(oy[i], -y, ey[i]);
+    }
+}
+
+// Tap `nth` at x (the w kernel of one plane), mirrored (wgridder.rs:1548-1560).
+template <class T, int S>
+__device__ __forceinline__ T eval_tap(const T* c, int nth, T x) {
+    constexpr int NE = ne_of<S>(), H = (S + 1) / 2;
+    if (nth >= H) {
+        nth = S - 1 - nth;
+        x = -x;
+    }
+    const T x2 = x * x;
+    T e = c[nth], o = c[NE * S + nth];
+#pragma unroll
+    for (int k = 1; k < NE; ++k) {
+        e = fma(e, x2, c[k * S + nth]);
+        o = fma(o, x2, c[(NE + k) * S + nth]);
+    }
+    return fma(o, x, e);
+}
+
+// w factor of a visibility at w on plane p, tap p - minplane (wgridder.rs:1580-1591).
+template <class T, int S>
+__device__ __forceinline__ T w_factor(const Geo& g, const T* c, int nth, int p, double w) {
+    const double w0 = g.wmin + double(p) * g.dw;
+    const T z = T(g.xdw * (w0 - w));
+    const T locz = (z - T(nth)) * T(2) + T(double(S) - 1.0);
+    return eval_tap<T, S>(c, nth, locz);
+}
+
+// ------------------------------------------------------------ gridding
+
+constexpr int kGridThreads = 256;
+
+// One workgroup per (tile, group of G planes). The accumulators of the
+// tile's cells on each plane of the group live in LDS; visibilities of this
+// tile and the spilling ones of the three preceding tiles add their taps that
+// fall inside it; the cells are written to the planes once. NOUT = 2 grids a
+// dirty image and a PSF together (wgridder.rs:2128-2152).
+template <class T, int S, int NOUT>
+__global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
+                                                       const std::uint32_t* __restrict__ off,
+                                                       const T* __restrict__ coeff, DevIn in,
+                                                       int G, cx<T>* __restrict__ grid) {
+    constexpr int NE = ne_of<S>();
+    __shared__ T coef[2 * NE * S];
+    extern __shared__ unsigned char smem[];
+    T* acc = reinterpret_cast<T*>(smem);
+    const int cells = g.Tu * g.Tv;
+    const int tile = blockIdx.x;
+    const int tu = tile / g.ntv, tv = tile - (tile / g.ntv) * g.ntv;
+    const int p0 = int(blockIdx.y) * G;
+    const int P = g.wgrid ? g.nplanes : 1;
+    const int p1 = min(P, p0 + G);
+    const int np = p1 - p0;
+    for (int e = threadIdx.x; e < NOUT * np * 2 * cells; e += kGridThreads) acc[e] = T(0);
+    for (int e = threadIdx.x; e < 2 * NE * S; e += kGridThreads) coef[e] = coeff[e];
+    __syncthreads();
+    const int mlo = g.wgrid ? max(0, p0 - S + 1) : 0;
+    const int mhi = g.wgrid ? min(g.nb - 1, p1 - 1) : 0;
+    for (int src = 0; src < 9; ++src) {
+        // (du, dv, class): own tile with every class, the tile before in u
+        // with classes spilling in u, before in v spilling in v, and the
+        // diagonal one spilling in both.
+        int du, dv, cat;
+        if (src < 4) {
+            du = 0, dv = 0, cat = src;
+        } else if (src < 6) {
+            du = 1, dv = 0, cat = 2 + (src - 4);
+        } else if (src < 8) {
+            du = 0, dv = 1, cat = 1 + 2 * (src - 6);
+        } else {
+            du = 1, dv = 1, cat = 3;
+        }
+        const int stile = ((tu - du + g.ntu) % g.ntu) * g.ntv + (tv - dv + g.ntv) % g.ntv;
+        const long long k0 = ((long long)stile * 4 + cat) * g.nb;
+        const std::uint32_t a = off[k0 + mlo], b = off[k0 + mhi + 1];
+        const int dou = du ? -g.Tu : 0, dov = dv ? -g.Tv : 0;
+        for (std::uint32_t r = a + threadIdx.x; r < b; r += kGridThreads) {
+            const Rec<T> rc = recs[r];
+            const int ou = int(rc.pos & 255u) + dou, ov = int((rc.pos >> 8) & 255u) + dov;
+            const int mp = int(rc.pos >> 16);
+            T ku[S], kv[S];
+            eval_taps2<T, S>(coef, rc.x0, rc.y0, ku, kv);
+            cx<T> val[NOUT];
+            if constexpr (NOUT == 2) {
+                const T fl = (rc.idx & kFlip) ? T(-1) : T(1);
+                const cx<T> d = cmul(vis_get<T>(in, rc.idx & ~kFlip), cx<T>{rc.re, rc.im * fl});
+                val[0] = {d.re, d.im * fl};
+                val[1] = {rc.re, rc.im};
+            } else {
+                val[0] = {rc.re, rc.im};
+            }
+            const int pa = g.wgrid ? max(p0, mp) : 0;
+            const int pb = g.wgrid ? min(p1, mp + S) : 1;
+            for (int p = pa; p < pb; ++p) {
+                T kw = T(1);
+                if (g.wgrid) kw = w_factor<T, S>(g, coef, p - mp, p, rc.w);
+#pragma unroll
+                for (int cu = 0; cu < S; ++cu) {
+                    const int iu = ou + cu;
+                    if (unsigned(iu) >= unsigned(g.Tu)) continue;
+                    const T kuc = g.wgrid ? ku[cu] * kw : ku[cu];
+#pragma unroll
+                    for (int o = 0; o < NOUT; ++o) {
+                        T* are = acc + ((o * np + (p - p0)) * 2) * cells + iu * g.Tv;
+                        T* aim = are + cells;
+                        const T tr = val[o].re * kuc, ti = val[o].im * kuc;
+#pragma unroll
+                        for (int cv = 0; cv < S; ++cv) {
+                            const int iv = ov + cv;
+                            if (unsigned(iv) >= unsigned(g.Tv)) continue;
+                            atomicAdd(are + iv, tr * kv[cv]);
+                            atomicAdd(aim + iv, ti * kv[cv]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const long long plane = (long long)g.nu * g.nv;
+    for (int e = threadIdx.x; e < np * cells; e += kGridThreads) {
+        const int pl = e / cells, rem = e - pl * cells;
+        const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
+        const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
+        const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+            const T* are = acc + ((o * np + pl) * 2) * cells;
+            grid[(long long)o * P * plane + (long long)(p0 + pl) * plane + (long long)grow * g.nv + gcol] =
+                cx<T>{are[rem], are[cells + rem]};
+        }
+    }
+}
+
+// ------------------------------------------------------------ transforms
+
+constexpr int kFftThreads = 256;
+
+template <class T>
+constexpr int fft_maxtot() {
+    return sizeof(T) == 4 ? 8192 : 4096;  // 64 KiB of complex values
+}
+
+// Per-pixel tables (wgridder.rs:806-861): n-1+nshift for the w-screens and
+// the global correction factor, on the quarter (or full with an l/m shift)
+// table of the reference.
+__device__ __forceinline__ double nm1_of(double x, double y, double r_l, double r_m) {
+    const double xsq = x * x, ysq = y * y, tmp = 1.0 - xsq - ysq;
+    const double nm1 = tmp >= 0.0 ? (-xsq - ysq) / (sqrt(tmp) + 1.0) : -sqrt(-tmp) - 1.0;
+    return nm1 + r_l * x + r_m * y;
+}
+
+__device__ __forceinline__ double corfunc(const double* glx, const double* glw, int ngl, double v) {
+    double t = 0.0;
+    for (int k = 0; k < ngl; ++k) t += glw[k] * cos(glx[k] * v);
+    return 1.0 / t;
+}
+
+__global__ __launch_bounds__(256) void k_tables(Geo g, double* screen, double* corr,
+                                                const double* cfu, const double* cfv,
+                                                const double* glx, const double* glw, int ngl) {
+    const long long total = (long long)g.tnx * g.tny;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int i = int(t / g.tny), j = int(t - (long long)i * g.tny);
+    const double x0 = g.lshift - 0.5 * double(g.nx) * g.pixsize_x;
+    const double y0 = g.mshift - 0.5 * double(g.ny) * g.pixsize_y;
+    const double x = x0 + double(i) * g.pixsize_x, y = y0 + double(j) * g.pixsize_y;
+    screen[t] = nm1_of(x, y, g.r_l, g.r_m) + g.nshift;
+    const double xsq = x * x, ysq = y * y, tmp = 1.0 - xsq - ysq;
+    double fct;
+    if (tmp >= 0.0) {
+        const double nm1 = (-xsq - ysq) / (sqrt(tmp) + 1.0);
+        fct = corfunc(glx, glw, ngl, (nm1 + g.r_l * x + g.r_m * y + g.nshift) * g.dw);
+        if (g.divide_by_n) fct /= nm1 + 1.0;
+    } else if (g.divide_by_n) {
+        fct = 0.0;
+    } else {
+        fct = corfunc(glx, glw, ngl, (-sqrt(-tmp) - 1.0 + g.nshift) * g.dw);
+    }
+    const int iu = g.nx / 2 - min(i, g.nx - i), iv = g.ny / 2 - min(j, g.ny - j);
+    corr[t] = fct * (cfu[iu] * cfv[iv]);
+}
+
+__device__ __forceinline__ int table_index(const Geo& g, int i, int j) {
+    if (g.lmshift) return i * g.tny + j;
+    return min(i, g.nx - i) * g.tny + min(j, g.ny - j);
+}
+
+// The w-screen of plane w at table value s (wgridder.rs:2296-2314).
+template <class T>
+__device__ __forceinline__ cx<T> screen_at(double w, double s, bool adjoint) {
+    double phs = w * s;
+    if (adjoint) phs = -phs;
+    T ang;
+    if constexpr (sizeof(T) == 4)
+        ang = T(2.0 * M_PI * (phs - floor(phs)));
+    else
+        ang = T(2.0 * M_PI * phs);
+    T sn, cs;
+    sincos_t(ang, sn, cs);
+    return {cs, sn};
+}
+
+// v2d: inverse transform along v of one grid row, keeping the image columns
+// (wgridder.rs:2331-2338, 1745-1749).
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void k_rows_inv(const cx<T>* __restrict__ grid,
+                                                          cx<T>* __restrict__ tmp, FftDesc d,
+                                                          const cx<T>* __restrict__ tw, int ny) {
+    extern __shared__ unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nv = d.n;
+    const long long row = blockIdx.x;
+    const cx<T>* src = grid + row * nv;
+    for (int e = threadIdx.x; e < nv; e += kFftThreads) buf[e] = src[e];
+    __syncthreads();
+    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, 1, tw, +1);
+    for (int j = threadIdx.x; j < ny; j += kFftThreads) tmp[row * ny + j] = buf[(nv - ny / 2 + j) % nv];
+}
+
+// v2d: inverse transform along u of C image columns of one plane, then the
+// projection Re(g screen) of the image rows (wgridder.rs:2342-2397), one term
+// per plane and pixel.
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void k_cols_inv(const cx<T>* __restrict__ tmp,
+                                                          T* __restrict__ proj, FftDesc d,
+                                                          const cx<T>* __restrict__ tw, Geo g,
+                                                          const double* __restrict__ screen, int C) {
+    extern __shared__ unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nu = d.n, ny = g.ny, nx = g.nx;
+    const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
+    const int p = blockIdx.y;
+    const cx<T>* src = tmp + (long long)p * nu * ny;
+    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) {
+        const int u = e / nc, c = e - u * nc;
+        buf[c * nu + u] = src[(long long)u * ny + j0 + c];
+    }
+    __syncthreads();
+    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, nc, tw, +1);
+    const double w0 = g.wmin + double(p) * g.dw;
+    T* dst = proj + (long long)p * nx * ny;
+    for (int e = threadIdx.x; e < nx * nc; e += kFftThreads) {
+        const int i = e / nc, c = e - i * nc, j = j0 + c;
+        const cx<T> gg = buf[c * nu + (nu - nx / 2 + i) % nu];
+        T term;
+        if (g.wgrid) {
+            const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], true);
+            term = gg.re * ph.re - gg.im * ph.im;
+        } else {
+            term = gg.re;
+        }
+        dst[(long long)i * ny + j] = term;
+    }
+}
+
+// v2d: the plane terms added in plane order, then the global correction
+// (wgridder.rs:2655-2659, 2403-2428).
+template <class T>
+__global__ __launch_bounds__(256) void k_sum(const T* __restrict__ proj, T* __restrict__ out, Geo g,
+                                             int P, const double* __restrict__ corr,
+                                             const double* __restrict__ cfu,
+                                             const double* __restrict__ cfv) {
+    const long long npix = (long long)g.nx * g.ny;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npix) return;
+    T acc = proj[e];
+    for (int p = 1; p < P; ++p) acc = acc + proj[(long long)p * npix + e];
+    const int i = int(e / g.ny), j = int(e - (long long)i * g.ny);
+    if (g.wgrid) {
+        out[e] = acc * T(corr[table_index(g, i, j)]);
+    } else {
+        const int iu = g.nx / 2 - min(i, g.nx - i), iv = g.ny / 2 - min(j, g.ny - j);
+        out[e] = T(double(acc) * cfu[iu] * cfv[iv]);
+    }
+}
+
+// d2v: the corrected copy of the image (wgridder.rs:3064-3091).
+template <class T>
+__global__ __launch_bounds__(256) void k_correct(const T* __restrict__ dirty, T* __restrict__ img,
+                                                 Geo g, const double* __restrict__ corr,
+                                                 const double* __restrict__ cfu,
+                                                 const double* __restrict__ cfv) {
+    const long long npix = (long long)g.nx * g.ny;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npix) return;
+    const int i = int(e / g.ny), j = int(e - (long long)i * g.ny);
+    if (g.wgrid) {
+        img[e] = dirty[e] * T(corr[table_index(g, i, j)]);
+    } else {
+        const int iu = g.nx / 2 - min(i, g.nx - i), iv = g.ny / 2 - min(j, g.ny - j);
+        img[e] = dirty[e] * T(cfu[iu] * cfv[iv]);
+    }
+}
+
+// d2v: C image columns times the plane's w-screen placed on the grid rows,
+// forward transform along u (wgridder.rs:2931-2967).
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void k_cols_fwd(const T* __restrict__ img,
+                                                          cx<T>* __restrict__ tmp, FftDesc d,
+                                                          const cx<T>* __restrict__ tw, Geo g,
+                                                          const double* __restrict__ screen, int C) {
+    extern __shared__ unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nu = d.n, ny = g.ny, nx = g.nx;
+    const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
+    const int p = blockIdx.y;
+    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) buf[e] = cx<T>{T(0), T(0)};
+    __syncthreads();
+    const double w0 = g.wmin + double(p) * g.dw;
+    for (int e = threadIdx.x; e < nx * nc; e += kFftThreads) {
+        const int i = e / nc, c = e - i * nc, j = j0 + c;
+        const T v = img[(long long)i * ny + j];
+        cx<T> val;
+        if (g.wgrid) {
+            const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], false);
+            val = {v * ph.re, v * ph.im};
+        } else {
+            val = {v, T(0)};
+        }
+        buf[c * nu + (nu - nx / 2 + i) % nu] = val;
+    }
+    __syncthreads();
+    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, nc, tw, -1);
+    cx<T>* dst = tmp + (long long)p * nu * ny;
+    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) {
+        const int u = e / nc, c = e - u * nc;
+        dst[(long long)u * ny + j0 + c] = buf[c * nu + u];
+    }
+}
+
+// d2v: one grid row from its image columns, forward transform along v
+// (wgridder.rs:2922-2929, 2968-2991).
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void k_rows_fwd(const cx<T>* __restrict__ tmp,
+                                                          cx<T>* __restrict__ grid, FftDesc d,
+                                                          const cx<T>* __restrict__ tw, int ny) {
+    extern __shared__ unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nv = d.n;
+    const long long row = blockIdx.x;
+    for (int e = threadIdx.x; e < nv; e += kFftThreads) buf[e] = cx<T>{T(0), T(0)};
+    __syncthreads();
+    for (int j = threadIdx.x; j < ny; j += kFftThreads) buf[(nv - ny / 2 + j) % nv] = tmp[row * ny + j];
+    __syncthreads();
+    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, 1, tw, -1);
+    cx<T>* dst = grid + row * nv;
+    for (int e = threadIdx.x; e < nv; e += kFftThreads) dst[e] = buf[e];
+}
+
+// d2v: degridding (wgridder.rs:2793-2901): per visibility of the tile, the
+// S x S taps of each of its planes against the plane grid, the planes added
+// in order, then the weight and shift phase (wgridder.rs:3146-3159).
+template <class T, int S>
+__global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict__ recs,
+                                                const std::uint32_t* __restrict__ off,
+                                                const T* __restrict__ coeff,
+                                                const cx<T>* __restrict__ grid,
+                                                cx<T>* __restrict__ vis) {
+    constexpr int NE = ne_of<S>();
+    __shared__ T coef[2 * NE * S];
+    for (int e = threadIdx.x; e < 2 * NE * S; e += blockDim.x) coef[e] = coeff[e];
+    __syncthreads();
+    const int tile = blockIdx.x;
+    const int tu = tile / g.ntv, tv = tile - (tile / g.ntv) * g.ntv;
+    const std::uint32_t a = off[(long long)tile * 4 * g.nb], b = off[(long long)(tile + 1) * 4 * g.nb];
+    const int P = g.wgrid ? g.nplanes : 1;
+    const long long plane = (long long)g.nu * g.nv;
+    for (std::uint32_t r = a + threadIdx.x; r < b; r += blockDim.x) {
+        const Rec<T> rc = recs[r];
+        const int ou = int(rc.pos & 255u), ov = int((rc.pos >> 8) & 255u), mp = int(rc.pos >> 16);
+        T ku[S], kv[S];
+        eval_taps2<T, S>(coef, rc.x0, rc.y0, ku, kv);
+        int rows[S], cols[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            rows[k] = (tu * g.Tu + ou + k + g.nu / 2) % g.nu;
+            cols[k] = (tv * g.Tv + ov + k + g.nv / 2) % g.nv;
+        }
+        const T fl = (rc.idx & kFlip) ? T(-1) : T(1);
+        cx<T> out = {T(0), T(0)};
+        const int pa = g.wgrid ? mp : 0, pb = g.wgrid ? min(P, mp + S) : 1;
+        for (int p = pa; p < pb; ++p) {
+            T kw = T(1);
+            if (g.wgrid) kw = w_factor<T, S>(g, coef, p - mp, p, rc.w);
+            const cx<T>* gp = grid + (long long)p * plane;
+            T rr = T(0), ri = T(0);
+#pragma unroll
+            for (int cu = 0; cu < S; ++cu) {
+                const T kuc = g.wgrid ? ku[cu] * kw : ku[cu];
+                const cx<T>* line = gp + (long long)rows[cu] * g.nv;
+                T tr = T(0), ti = T(0);
+#pragma unroll
+                for (int cv = 0; cv < S; ++cv) {
+                    const cx<T> v = line[cols[cv]];
+                    tr = fma(kv[cv], v.re, tr);
+                    ti = fma(kv[cv], v.im, ti);
+                }
+                rr = fma(kuc, tr, rr);
+                ri = fma(kuc, ti, ri);
+            }
+            out = {rr + out.re, ri * fl + out.im};
+        }
+        cx<T> res;
+        if (g.shifting)
+            res = cmul(out, cx<T>{rc.re, rc.im});
+        else
+            res = {out.re * rc.re, out.im * rc.re};
+        vis[rc.idx & ~kFlip] = res;
+    }
+}
+
+// ------------------------------------------------------------ dispatch
+
+template <class T>
+constexpr int max_support() {
+    return sizeof(T) == 4 ? 8 : 16;
+}
+
+template <class T, int NOUT>
+void launch_grid(int S, dim3 grid, std::size_t lds, hipStream_t st, const Geo& g,
+                 const Rec<T>* recs, const std::uint32_t* off, const T* coeff, const DevIn& in,
+                 int G, cx<T>* out) {
+#define GRID_CASE(SS)                                                                        \
+    case SS:                                                                                 \
+        if constexpr (SS <= max_support<T>()) {                                              \
+            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT>,                     \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))); \
+            k_grid<T, SS, NOUT><<<grid, kGridThreads, lds, st>>>(g, recs, off, coeff, in, G, out); \
+        } else {                                                                             \
+            throw DevError{"kernel support too large for this precision"};                   \
+        }                                                                                    \
+        break;
+    switch (S) {
+        GRID_CASE(4) GRID_CASE(5) GRID_CASE(6) GRID_CASE(7) GRID_CASE(8) GRID_CASE(9)
+        GRID_CASE(10) GRID_CASE(11) GRID_CASE(12) GRID_CASE(13) GRID_CASE(14) GRID_CASE(15)
+        GRID_CASE(16)
+    default:
+        throw DevError{"unsupported kernel support"};
+    }
+#undef GRID_CASE
+    HIPCHK(hipGetLastError());
+}
+
+template <class T>
+void launch_degrid(int S, int ntiles, hipStream_t st, const Geo& g, const Rec<T>* recs,
+                   const std::uint32_t* off, const T* coeff, const cx<T>* grid, cx<T>* vis) {
+#define DEGRID_CASE(SS)                                                                   \
+    case SS:                                                                              \
+        if constexpr (SS <= max_support<T>())                                             \
+            k_degrid<T, SS><<<ntiles, 256, 0, st>>>(g, recs, off, coeff, grid, vis);      \
+        else                                                                              \
+            throw DevError{"kernel support too large for this precision"};                \
+        break;
+    switch (S) {
+        DEGRID_CASE(4) DEGRID_CASE(5) DEGRID_CASE(6) DEGRID_CASE(7) DEGRID_CASE(8) DEGRID_CASE(9)
+        DEGRID_CASE(10) DEGRID_CASE(11) DEGRID_CASE(12) DEGRID_CASE(13) DEGRID_CASE(14)
+        DEGRID_CASE(15) DEGRID_CASE(16)
+    default:
+        throw DevError{"unsupported kernel support"};
+    }
+#undef DEGRID_CASE
+    HIPCHK(hipGetLastError());
+}
+
+// Tile side along a grid axis of n cells: a divisor of n (so that no tile
+// is partial) of at least the support and at most `hi`, nearest 32.
+int choose_tile(int n, int supp, int hi) {
+    int best = -1;
+    for (int t = std::max(supp, 8); t <= std::min(hi, n); ++t)
+        if (n % t == 0 && (best < 0 || std::abs(t - 32) < std::abs(best - 32))) best = t;
+    if (best < 0 && n <= 255) best = n;
+    return best;
+}
+
+void exclusive_scan(Workspace& ws, const std::uint32_t* in, std::uint32_t* out, long long n,
+                    hipStream_t st) {
+    // level sizes
+    std::vector<long long> sizes{n};
+    while (sizes.back() > kScanItems) sizes.push_back((sizes.back() + kScanItems - 1) / kScanItems);
+    long long total = 0;
+    for (std::size_t i = 1; i < sizes.size(); ++i) total += 2 * sizes[i];
+    std::uint32_t* tmp = static_cast<std::uint32_t*>(ws.get(kSlotScan, sizeof(std::uint32_t) * (total + 2)));
+    std::vector<std::uint32_t*> sums, sums_scanned;
+    std::uint32_t* p = tmp;
+    for (std::size_t i = 1; i < sizes.size(); ++i) {
+        sums.push_back(p);
+        p += sizes[i];
+        sums_scanned.push_back(p);
+        p += sizes[i];
+    }
+    std::vector<const std::uint32_t*> ins{in};
+    std::vector<std::uint32_t*> outs{out};
+    for (std::size_t i = 1; i < sizes.size(); ++i) {
+        ins.push_back(sums[i - 1]);
+        outs.push_back(sums_scanned[i - 1]);
+    }
+    std::uint32_t* dummy = tmp + total;  // one slot for the top level's total
+    for (std::size_t lv = 0; lv < sizes.size(); ++lv) {
+        const long long m = sizes[lv];
+        const long long nblk = (m + kScanItems - 1) / kScanItems;
+        std::uint32_t* s = lv + 1 < sizes.size() ? sums[lv] : dummy;
+        k_scan_block<<<nblk, 1024, 0, st>>>(ins[lv], outs[lv], s, m);
+        HIPCHK(hipGetLastError());
+    }
+    for (std::size_t lv = sizes.size() - 1; lv-- > 0;) {
+        const long long m = sizes[lv];
+        k_scan_add<<<(m + 255) / 256, 256, 0, st>>>(outs[lv], outs[lv + 1], m);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+int grid_blocks(long long n) {
+    return int(std::min<long long>((n + 255) / 256, 256 * 64));
+}
+
+template <class T>
+struct Tables {
+    FftDesc du, dv;
+    cx<T>* twu = nullptr;
+    cx<T>* twv = nullptr;
+    T* coeff = nullptr;
+    double* screen = nullptr;
+    double* corr = nullptr;
+    double* cfu = nullptr;
+    double* cfv = nullptr;
+};
+
+template <class T>
+void upload(Workspace& ws, int slot, const std::vector<T>& h, hipStream_t st, T** out) {
+    *out = static_cast<T*>(ws.get(slot, sizeof(T) * std::max<std::size_t>(h.size(), 1)));
+    HIPCHK(hipMemcpyAsync(*out, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+}
+
+// Host-side values kept alive until the call's stream is synchronised.
+struct HostKeep {
+    std::vector<std::vector<double>> d;
+    std::vector<std::vector<float>> f;
+};
+
+template <class T>
+std::vector<T> cast_vec(const std::vector<double>& v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+template <class T>
+void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st, HostKeep& keep,
+                 Tables<T>& tb) {
+    std::vector<double> twu, twv;
+    if (!fft_describe(int(G.nu), tb.du, twu) || !fft_describe(int(G.nv), tb.dv, twv))
+        throw DevError{"grid size is not a product of 2, 3, 5, 7 and 11"};
+    auto upload_cx = [&](int slot, const std::vector<double>& h, cx<T>** out) {
+        if constexpr (sizeof(T) == 8) {
+            keep.d.push_back(h);
+            *out = static_cast<cx<T>*>(ws.get(slot, sizeof(double) * h.size()));
+            HIPCHK(hipMemcpyAsync(*out, keep.d.back().data(), sizeof(double) * h.size(),
+                                  hipMemcpyHostToDevice, st));
+        } else {
+            keep.f.push_back(cast_vec<float>(h));
+            *out = static_cast<cx<T>*>(ws.get(slot, sizeof(float) * h.size()));
+            HIPCHK(hipMemcpyAsync(*out, keep.f.back().data(), sizeof(float) * h.size(),
+                                  hipMemcpyHostToDevice, st));
+        }
+    };
+    upload_cx(kSlotTwU, twu, &tb.twu);
+    upload_cx(kSlotTwV, twv, &tb.twv);
+    int ne = 0;
+    std::vector<double> eo = G.kernel.even_odd(ne);
+    if constexpr (sizeof(T) == 8) {
+        keep.d.push_back(eo);
+        tb.coeff = static_cast<T*>(ws.get(kSlotCoeff, sizeof(double) * eo.size()));
+        HIPCHK(hipMemcpyAsync(tb.coeff, keep.d.back().data(), sizeof(double) * eo.size(),
+                              hipMemcpyHostToDevice, st));
+    } else {
+        keep.f.push_back(cast_vec<float>(eo));
+        tb.coeff = static_cast<T*>(ws.get(kSlotCoeff, sizeof(float) * eo.size()));
+        HIPCHK(hipMemcpyAsync(tb.coeff, keep.f.back().data(), sizeof(float) * eo.size(),
+                              hipMemcpyHostToDevice, st));
+    }
+    // corfunc tables along u and v (wgridder.rs:824-829)
+    std::vector<double> cfu(G.nx / 2 + 1), cfv(G.ny / 2 + 1);
+    for (std::size_t i = 0; i < cfu.size(); ++i) cfu[i] = G.kernel.corfunc(double(i) * (1.0 / double(G.nu)));
+    for (std::size_t i = 0; i < cfv.size(); ++i) cfv[i] = G.kernel.corfunc(double(i) * (1.0 / double(G.nv)));
+    keep.d.push_back(cfu);
+    tb.cfu = static_cast<double*>(ws.get(kSlotCfu, sizeof(double) * cfu.size()));
+    HIPCHK(hipMemcpyAsync(tb.cfu, keep.d.back().data(), sizeof(double) * cfu.size(), hipMemcpyHostToDevice, st));
+    keep.d.push_back(cfv);
+    tb.cfv = static_cast<double*>(ws.get(kSlotCfv, sizeof(double) * cfv.size()));
+    HIPCHK(hipMemcpyAsync(tb.cfv, keep.d.back().data(), sizeof(double) * cfv.size(), hipMemcpyHostToDevice, st));
+    if (g.wgrid) {
+        std::vector<double> gl(G.kernel.corr_x);
+        gl.insert(gl.end(), G.kernel.corr_w.begin(), G.kernel.corr_w.end());
+        keep.d.push_back(gl);
+        double* dgl = static_cast<double*>(ws.get(kSlotGL, sizeof(double) * gl.size()));
+        HIPCHK(hipMemcpyAsync(dgl, keep.d.back().data(), sizeof(double) * gl.size(), hipMemcpyHostToDevice, st));
+        const long long tn = (long long)g.tnx * g.tny;
+        tb.screen = static_cast<double*>(ws.get(kSlotScreen, sizeof(double) * tn));
+        tb.corr = static_cast<double*>(ws.get(kSlotCorr, sizeof(double) * tn));
+        const int ngl = int(G.kernel.corr_x.size());
+        k_tables<<<(tn + 255) / 256, 256, 0, st>>>(g, tb.screen, tb.corr, tb.cfu, tb.cfv, dgl, dgl + ngl, ngl);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+// The part of both directions before the transforms: foc upload, scan,
+// geometry, keys, sort, records. Returns false (with info zeroed) when no
+// visibility is active.
+template <class T>
+struct Prepared {
+    Geo g;
+    Geometry G;
+    Rec<T>* recs = nullptr;
+    std::uint32_t* off = nullptr;
+    long long nvis = 0;
+};
+
+template <class T, bool USE_VIS>
+unsigned long long run_scan(Workspace& ws, const DevIn& din, const Geo& g, hipStream_t st,
+                            double& wlo, double& whi) {
+    if (g.n == 0) return 0;
+    unsigned long long* cnt = static_cast<unsigned long long*>(ws.get(kSlotCounters, 3 * sizeof(unsigned long long)));
+    unsigned long long init[3] = {0ull, 0x7ff0000000000000ull, 0ull};
+    HIPCHK(hipMemcpyAsync(cnt, init, sizeof init, hipMemcpyHostToDevice, st));
+    k_scan<T, USE_VIS><<<grid_blocks(g.n), 256, 0, st>>>(din, g, cnt);
+    HIPCHK(hipGetLastError());
+    unsigned long long res[3];
+    HIPCHK(hipMemcpyAsync(res, cnt, sizeof res, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::memcpy(&wlo, &res[1], sizeof(double));
+    std::memcpy(&whi, &res[2], sizeof(double));
+    return res[0];
+}
+
+// Geometry and tiling into the kernels' Geo.
+template <class T>
+bool fill_geo(const Geometry& G, Geo& g, int nout, std::string& err) {
+    g.nx = int(G.nx);
+    g.ny = int(G.ny);
+    g.nu = int(G.nu);
+    g.nv = int(G.nv);
+    g.supp = G.supp;
+    g.pixsize_x = G.pixsize_x;
+    g.pixsize_y = G.pixsize_y;
+    g.ushift = G.ushift;
+    g.vshift = G.vshift;
+    g.maxiu0 = G.maxiu0;
+    g.maxiv0 = G.maxiv0;
+    g.wmin = G.wmin;
+    g.dw = G.dw;
+    g.xdw = G.xdw;
+    g.wshift = G.wshift;
+    g.wgrid = G.do_wgridding ? 1 : 0;
+    g.nplanes = int(G.nplanes);
+    g.nb = int(std::max<std::size_t>(G.nplanes, 1));
+    g.r_l = G.r_l;
+    g.r_m = G.r_m;
+    g.lshift = G.lshift;
+    g.mshift = G.mshift;
+    g.nshift = G.nshift;
+    g.shifting = G.shifting ? 1 : 0;
+    g.lmshift = G.lmshift ? 1 : 0;
+    g.divide_by_n = G.divide_by_n ? 1 : 0;
+    g.tnx = G.lmshift ? g.nx : g.nx / 2 + 1;
+    g.tny = G.lmshift ? g.ny : g.ny / 2 + 1;
+    // tile sides: LDS of one plane of nout outputs within 40 KiB
+    const int hi = int(std::sqrt(40960.0 / (double(nout) * 2.0 * sizeof(T))));
+    g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64));
+    g.Tv = choose_tile(g.nv, g.supp, std::min(hi, 64));
+    if (g.Tu < 0 || g.Tv < 0) {
+        err = "no tile size divides the grid";
+        return false;
+    }
+    g.ntu = g.nu / g.Tu;
+    g.ntv = g.nv / g.Tv;
+    const long long nkeys = (long long)g.ntu * g.ntv * 4 * g.nb;
+    if (nkeys >= 0xffffffffll) {
+        err = "grid too large for one call";
+        return false;
+    }
+    if (std::max(g.nu, g.nv) > fft_maxtot<T>()) {
+        err = "grid dimension above the in-LDS transform limit of this build";
+        return false;
+    }
+    return true;
+}
+
+template <class T, int MODE, bool USE_VIS>
+void sort_records(Workspace& ws, const DevIn& din, const Geo& g, long long nvis, hipStream_t st,
+                  Rec<T>** recs_out, std::uint32_t** off_out) {
+    const long long nkeys = (long long)g.ntu * g.ntv * 4 * g.nb;
+    std::uint32_t* keys = static_cast<std::uint32_t*>(ws.get(kSlotKeys, sizeof(std::uint32_t) * g.n));
+    std::uint32_t* counts = static_cast<std::uint32_t*>(ws.get(kSlotCounts, sizeof(std::uint32_t) * (nkeys + 1)));
+    std::uint32_t* off = static_cast<std::uint32_t*>(ws.get(kSlotOffsets, sizeof(std::uint32_t) * (nkeys + 1)));
+    std::uint32_t* cursor = static_cast<std::uint32_t*>(ws.get(kSlotCursor, sizeof(std::uint32_t) * (nkeys + 1)));
+    Rec<T>* recs = static_cast<Rec<T>*>(ws.get(kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
+    HIPCHK(hipMemsetAsync(counts, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
+    HIPCHK(hipMemsetAsync(cursor, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
+    k_keys<T, USE_VIS><<<grid_blocks(g.n), 256, 0, st>>>(din, g, keys, counts);
+    HIPCHK(hipGetLastError());
+    exclusive_scan(ws, counts, off, nkeys + 1, st);
+    k_scatter<T, MODE><<<grid_blocks(g.n), 256, 0, st>>>(din, g, keys, off, cursor, recs);
+    HIPCHK(hipGetLastError());
+    *recs_out = recs;
+    *off_out = off;
+}
+
+Geo base_geo(const DevIn& din, std::size_t nrow, std::size_t nchan, const Options& o) {
+    Geo g{};
+    g.n = (long long)(nrow * nchan);
+    g.nrow = int(nrow);
+    g.nchan = int(nchan);
+    double r_l, r_m;
+    recentring(o, r_l, r_m);
+    g.r_l = r_l;
+    g.r_m = r_m;
+    g.fu = o.flip_u ? -1.0 : 1.0;
+    g.fv = o.flip_v ? -1.0 : 1.0;
+    g.fw = !o.flip_w ? -1.0 : 1.0;  // the core's w flip (wgridder.rs:2461)
+    (void)din;
+    return g;
+}
+
+int device_of() {
+    int d = 0;
+    HIPCHK(hipGetDevice(&d));
+    return d;
+}
+
+bool common_checks(const Options& o, const DevInputs& in, const double* freq_host, std::string& err) {
+    if (!check_common(o, in.nrow, in.nchan, err)) return false;
+    if (in.nrow * in.nchan >= (std::size_t(1) << 31)) {
+        err = "too many visibilities for one call (at most 2^31 - 1)";
+        return false;
+    }
+    if (!check_freq(freq_host, in.nchan, err)) return false;
+    return true;
+}
+
+double* upload_foc(Workspace& ws, const double* freq_host, std::size_t nchan, hipStream_t st,
+                   HostKeep& keep) {
+    std::vector<double> foc(nchan);
+    for (std::size_t c = 0; c < nchan; ++c) foc[c] = freq_host[c] / kCLight;
+    keep.d.push_back(std::move(foc));
+    double* d = static_cast<double*>(ws.get(kSlotFoc, sizeof(double) * nchan));
+    HIPCHK(hipMemcpyAsync(d, keep.d.back().data(), sizeof(double) * nchan, hipMemcpyHostToDevice, st));
+    return d;
+}
+
+}  // namespace
+
+void set_stage_timing(bool on) { g_timing = on; }
+int stage_timing_count() { return int(g_times.size()); }
+const char* stage_timing_name(int i) {
+    return i >= 0 && i < int(g_times.size()) ? g_times[i].first.c_str() : "";
+}
+double stage_timing_ms(int i) { return i >= 0 && i < int(g_times.size()) ? g_times[i].second : 0.0; }
+
+template <class T>
+int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
+                     T* psf, double* info, hipStream_t st, std::string& err) {
+    try {
+        if (!dirty && !psf) {
+            err = "null array";
+            return 1;
+        }
+        if (!common_checks(o, in, freq_host, err)) return 1;
+        Workspace& ws = workspace(device_of());
+        std::lock_guard<std::mutex> lk(ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        DevIn din{in.uvw, nullptr, in.vis, in.wgt, in.wgt2, in.mask, in.vis_kind, in.wgt_kind};
+        Geo g = base_geo(din, in.nrow, in.nchan, o);
+        din.foc = upload_foc(ws, freq_host, in.nchan, st, keep);
+        const bool use_vis = psf == nullptr;
+        double wlo = 0, whi = 0;
+        const unsigned long long nvis = use_vis ? run_scan<T, true>(ws, din, g, st, wlo, whi)
+                                                : run_scan<T, false>(ws, din, g, st, wlo, whi);
+        timer.mark("scan");
+        const std::size_t npix = o.nx * o.ny;
+        if (dirty) HIPCHK(hipMemsetAsync(dirty, 0, sizeof(T) * npix, st));
+        if (psf) HIPCHK(hipMemsetAsync(psf, 0, sizeof(T) * npix, st));
+        std::fill(info, info + kNInfo, 0.0);
+        if (nvis == 0) {
+            HIPCHK(hipStreamSynchronize(st));
+            return 0;
+        }
+        Geometry G;
+        if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
+        const int nout = (dirty ? 1 : 0) + (psf ? 1 : 0);
+        if (!fill_geo<T>(G, g, nout, err)) return 1;
+        Rec<T>* recs;
+        std::uint32_t* off;
+        if (use_vis)
+            sort_records<T, 0, true>(ws, din, g, (long long)nvis, st, &recs, &off);
+        else
+            sort_records<T, 1, false>(ws, din, g, (long long)nvis, st, &recs, &off);
+        timer.mark("sort");
+        Tables<T> tb;
+        make_tables<T>(ws, G, g, st, keep, tb);
+        timer.mark("tables");
+        const int P = g.wgrid ? g.nplanes : 1;
+        const long long plane = (long long)g.nu * g.nv;
+        cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
+        const std::size_t per_plane = std::size_t(nout) * g.Tu * g.Tv * 2 * sizeof(T);
+        if (per_plane > 48 * 1024) throw DevError{"tile too large for LDS"};
+        const int G_planes = std::max(1, std::min(P, int((48 * 1024) / per_plane)));
+        const dim3 ggrid(g.ntu * g.ntv, (P + G_planes - 1) / G_planes);
+        const std::size_t lds = per_plane * std::min(P, G_planes);
+        if (nout == 2)
+            launch_grid<T, 2>(g.supp, ggrid, lds, st, g, recs, off, tb.coeff, din, G_planes, grid);
+        else
+            launch_grid<T, 1>(g.supp, ggrid, lds, st, g, recs, off, tb.coeff, din, G_planes, grid);
+        timer.mark("grid");
+        cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
+        T* proj = static_cast<T*>(ws.get(kSlotProj, sizeof(T) * (long long)P * npix));
+        const int C = std::max(1, std::min(4, fft_maxtot<T>() / g.nu));
+        T* outs[2] = {dirty ? dirty : psf, psf};
+        for (int oi = 0; oi < nout; ++oi) {
+            const cx<T>* gsrc = grid + (long long)oi * P * plane;
+            const std::size_t ldsv = sizeof(cx<T>) * g.nv, ldsu = sizeof(cx<T>) * g.nu * C;
+            HIPCHK(hipFuncSetAttribute((const void*)k_rows_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsv)));
+            k_rows_inv<T><<<(long long)P * g.nu, kFftThreads, ldsv, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+            HIPCHK(hipGetLastError());
+            timer.mark("fft_v");
+            HIPCHK(hipFuncSetAttribute((const void*)k_cols_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsu)));
+            k_cols_inv<T><<<dim3((g.ny + C - 1) / C, P), kFftThreads, ldsu, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+            HIPCHK(hipGetLastError());
+            timer.mark("fft_u");
+            k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
+            HIPCHK(hipGetLastError());
+            timer.mark("sum");
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        std::copy(G.info, G.info + kNInfo, info);
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template <class T>
+int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
+                     const T* dirty, T* vis, double* info, hipStream_t st, std::string& err) {
+    try {
+        if (!dirty || !vis) {
+            err = "null array";
+            return 1;
+        }
+        if (!common_checks(o, in, freq_host, err)) return 1;
+        Workspace& ws = workspace(device_of());
+        std::lock_guard<std::mutex> lk(ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        DevIn din{in.uvw, nullptr, nullptr, in.wgt, nullptr, in.mask, kVisUnit, in.wgt_kind};
+        Geo g = base_geo(din, in.nrow, in.nchan, o);
+        din.foc = upload_foc(ws, freq_host, in.nchan, st, keep);
+        double wlo = 0, whi = 0;
+        const unsigned long long nvis = run_scan<T, false>(ws, din, g, st, wlo, whi);
+        timer.mark("scan");
+        HIPCHK(hipMemsetAsync(vis, 0, sizeof(T) * 2 * in.nrow * in.nchan, st));
+        std::fill(info, info + kNInfo, 0.0);
+        if (nvis == 0) {
+            HIPCHK(hipStreamSynchronize(st));
+            return 0;
+        }
+        Geometry G;
+        if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
+        if (!fill_geo<T>(G, g, 1, err)) return 1;
+        Rec<T>* recs;
+        std::uint32_t* off;
+        sort_records<T, 2, false>(ws, din, g, (long long)nvis, st, &recs, &off);
+        timer.mark("sort");
+        Tables<T> tb;
+        make_tables<T>(ws, G, g, st, keep, tb);
+        timer.mark("tables");
+        const int P = g.wgrid ? g.nplanes : 1;
+        const long long plane = (long long)g.nu * g.nv;
+        const std::size_t npix = o.nx * o.ny;
+        T* img = static_cast<T*>(ws.get(kSlotImg, sizeof(T) * npix));
+        k_correct<T><<<(npix + 255) / 256, 256, 0, st>>>(dirty, img, g, tb.corr, tb.cfu, tb.cfv);
+        HIPCHK(hipGetLastError());
+        cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
+        cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
+        const int C = std::max(1, std::min(4, fft_maxtot<T>() / g.nu));
+        const std::size_t ldsv = sizeof(cx<T>) * g.nv, ldsu = sizeof(cx<T>) * g.nu * C;
+        HIPCHK(hipFuncSetAttribute((const void*)k_cols_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsu)));
+        k_cols_fwd<T><<<dim3((g.ny + C - 1) / C, P), kFftThreads, ldsu, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+        HIPCHK(hipGetLastError());
+        timer.mark("fft_u");
+        HIPCHK(hipFuncSetAttribute((const void*)k_rows_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsv)));
+        k_rows_fwd<T><<<(long long)P * g.nu, kFftThreads, ldsv, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
+        HIPCHK(hipGetLastError());
+        timer.mark("fft_v");
+        launch_degrid<T>(g.supp, g.ntu * g.ntv, st, g, recs, off, tb.coeff, grid, reinterpret_cast<cx<T>*>(vis));
+        timer.mark("degrid");
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        std::copy(G.info, G.info + kNInfo, info);
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template int device_vis2dirty<float>(const Options&, const DevInputs&, const double*, float*,
+                                     float*, double*, hipStream_t, std::string&);
+template int device_vis2dirty<double>(const Options&, const DevInputs&, const double*, double*,
+                                      double*, double*, hipStream_t, std::string&);
+template int device_dirty2vis<float>(const Options&, const DevInputs&, const double*,
+                                     const float*, float*, double*, hipStream_t, std::string&);
+template int device_dirty2vis<double>(const Options&, const DevInputs&, const double*,
+                                      const double*, double*, double*, hipStream_t, std::string&);
+
+}  // namespace rang

@@ -1,0 +1,266 @@
+"""The gridder slice of rang-core's extension module `rang._core`, on the
+MI355X library.
+
+Same function names, argument order, checks and return values as the
+reference's PyO3 bindings (core/py/src/lib.rs:1055-1446): option lists of 18
+floats, NumPy buffers borrowed in place, status codes returned. The calls go
+through the C ABI of include/rang_core_wgridder.h (ctypes), whose work runs on
+the GPU. There is no CPU fallback: without the built library every call
+raises.
+"""
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parents[1] / "lib" / "librang_core_amd.so"
+N_INFO = 14
+
+VIS_UNIT, VIS_C64, VIS_C128 = 0, 1, 2
+WGT_NONE, WGT_F32, WGT_F64, WGT_F32_PRODUCT = 0, 1, 2, 3
+
+
+class WgridderOptions(ctypes.Structure):
+    """core/src/ffi_wgridder.rs:22-43."""
+
+    _fields_ = [
+        ("nx", ctypes.c_size_t),
+        ("ny", ctypes.c_size_t),
+        ("pixsize_x", ctypes.c_double),
+        ("pixsize_y", ctypes.c_double),
+        ("epsilon", ctypes.c_double),
+        ("center_x", ctypes.c_double),
+        ("center_y", ctypes.c_double),
+        ("sigma_min", ctypes.c_double),
+        ("sigma_max", ctypes.c_double),
+        ("do_wgridding", ctypes.c_int32),
+        ("divide_by_n", ctypes.c_int32),
+        ("flip_u", ctypes.c_int32),
+        ("flip_v", ctypes.c_int32),
+        ("flip_w", ctypes.c_int32),
+        ("allow_nshift", ctypes.c_int32),
+        ("kernel_index", ctypes.c_int64),
+        ("nu", ctypes.c_size_t),
+        ("nv", ctypes.c_size_t),
+    ]
+
+
+# Every symbol of include/rang_core_wgridder.h with its ctypes signature.
+_vp, _sz, _i32, _dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_double
+_OPT = ctypes.POINTER(WgridderOptions)
+SIGNATURES = {
+    "rang_core_wgridder_vis2dirty_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_vis2dirty_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_dirty2vis_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_dirty2vis_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_vis2dirty_ex_f64": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_vis2dirty_ex_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_vis2dirty_ex2_f64": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_vis2dirty_ex2_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_dirty2vis_ex_f64": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_dirty2vis_ex_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp], _i32),
+    "rang_core_wgridder_kernel_row": ([_sz, _vp], _i32),
+    "rang_core_wgridder_last_error": ([_vp, _sz], _sz),
+    "rang_amd_wgridder_vis2dirty_dev_f64": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_vis2dirty_dev_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_dirty2vis_dev_f64": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_dirty2vis_dev_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_stage_timing": ([_i32], None),
+    "rang_amd_stage_count": ([], _i32),
+    "rang_amd_stage_name": ([_i32], ctypes.c_char_p),
+    "rang_amd_stage_ms": ([_i32], _dp),
+}
+
+_lib = None
+
+
+def library():
+    """The loaded C-ABI library; raises when it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"the MI355X gridder library is not built ({LIB_PATH}); "
+                "run __graft_entry__.build() or make -C rang-core_amd"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def library_path():
+    return str(LIB_PATH)
+
+
+def options(o):
+    """`WgridderOptions` from the 18 option floats in declaration order
+    (core/py/src/lib.rs:1055-1080)."""
+    if len(o) != 18:
+        raise ValueError("18 gridder options required")
+    return WgridderOptions(
+        nx=int(o[0]), ny=int(o[1]), pixsize_x=o[2], pixsize_y=o[3], epsilon=o[4],
+        center_x=o[5], center_y=o[6], sigma_min=o[7], sigma_max=o[8],
+        do_wgridding=int(o[9]), divide_by_n=int(o[10]), flip_u=int(o[11]), flip_v=int(o[12]),
+        flip_w=int(o[13]), allow_nshift=int(o[14]), kernel_index=int(o[15]), nu=int(o[16]),
+        nv=int(o[17]),
+    )
+
+
+def _contig(a, name):
+    """The reference refuses non-C-contiguous arrays (core/py/src/lib.rs:113-137)."""
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{name} must be C-contiguous")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _vis_arg(vis):
+    if vis is None:
+        return None, VIS_UNIT
+    vis = np.asarray(vis)
+    if vis.dtype == np.complex64:
+        return _contig(vis, "vis"), VIS_C64
+    if vis.dtype == np.complex128:
+        return _contig(vis, "vis"), VIS_C128
+    raise ValueError("vis must be complex64 or complex128")
+
+
+def _wgt_arg(wgt):
+    if wgt is None:
+        return None, WGT_NONE
+    wgt = np.asarray(wgt)
+    if wgt.dtype == np.float32:
+        return _contig(wgt, "wgt"), WGT_F32
+    if wgt.dtype == np.float64:
+        return _contig(wgt, "wgt"), WGT_F64
+    raise ValueError("weights must be float32 or float64")
+
+
+def _checked(a, dtype, name):
+    if a is None:
+        return None
+    if not isinstance(a, np.ndarray) or a.dtype != dtype:
+        raise TypeError(f"{name} must be a {np.dtype(dtype).name} array")
+    return _contig(a, name)
+
+
+def _v2d(opts, uvw, freq, vis, wgt, wgt2, mask, dirty, psf, info, real, fn):
+    """core/py/src/lib.rs:1224-1280 (wgridder_v2d)."""
+    o = options(opts)
+    uvw = _checked(uvw, np.float64, "uvw")
+    freq = _checked(freq, np.float64, "freq")
+    nrow, nchan = uvw.size // 3, freq.size
+    n, npix = nrow * nchan, o.nx * o.ny
+    vis, vis_kind = _vis_arg(vis)
+    wgt, wgt_kind = _wgt_arg(wgt)
+    wgt2 = _checked(wgt2, np.float32, "wgt2")
+    if wgt2 is not None:
+        if wgt_kind != WGT_F32:
+            raise ValueError("two weights must both be float32")
+        wgt_kind = WGT_F32_PRODUCT
+    mask = _checked(mask, np.uint8, "mask")
+    dirty = _checked(dirty, real, "dirty")
+    psf = _checked(psf, real, "psf")
+    info = _checked(info, np.float64, "info")
+    ok = (
+        uvw.size == nrow * 3
+        and (vis is None or vis.size == n)
+        and (wgt is None or wgt.size == n)
+        and (wgt2 is None or wgt2.size == n)
+        and (mask is None or mask.size == n)
+        and (dirty is None or dirty.size == npix)
+        and (psf is None or psf.size == npix)
+        and info.size == N_INFO
+    )
+    if not ok:
+        raise ValueError("gridder array sizes disagree")
+    return int(fn(ctypes.byref(o), nrow, nchan, _ptr(uvw), _ptr(freq), _ptr(vis), vis_kind,
+                  _ptr(wgt), _ptr(wgt2), wgt_kind, _ptr(mask), _ptr(dirty), _ptr(psf), _ptr(info)))
+
+
+def wgridder_vis2dirty_f64(opts, uvw, freq, vis, wgt, wgt2, mask, dirty, psf, info):
+    """`rang_core_wgridder_vis2dirty_ex2_f64`: data of either complex precision
+    and weights of either real precision read in place; float64 outputs;
+    returns the status code (core/py/src/lib.rs:1283-1318)."""
+    return _v2d(opts, uvw, freq, vis, wgt, wgt2, mask, dirty, psf, info, np.float64,
+                library().rang_core_wgridder_vis2dirty_ex2_f64)
+
+
+def wgridder_vis2dirty_f32(opts, uvw, freq, vis, wgt, wgt2, mask, dirty, psf, info):
+    """As `wgridder_vis2dirty_f64`, working in single precision
+    (core/py/src/lib.rs:1320-1355)."""
+    return _v2d(opts, uvw, freq, vis, wgt, wgt2, mask, dirty, psf, info, np.float32,
+                library().rang_core_wgridder_vis2dirty_ex2_f32)
+
+
+def _d2v(opts, uvw, freq, dirty, wgt, mask, vis, info, real, fn):
+    """core/py/src/lib.rs:1357-1395 (wgridder_d2v); `vis` is the real view."""
+    o = options(opts)
+    uvw = _checked(uvw, np.float64, "uvw")
+    freq = _checked(freq, np.float64, "freq")
+    nrow, nchan = uvw.size // 3, freq.size
+    n = nrow * nchan
+    dirty = _checked(dirty, real, "dirty")
+    wgt, wgt_kind = _wgt_arg(wgt)
+    mask = _checked(mask, np.uint8, "mask")
+    vis = _checked(vis, real, "vis")
+    info = _checked(info, np.float64, "info")
+    ok = (
+        uvw.size == nrow * 3
+        and dirty.size == o.nx * o.ny
+        and vis.size == 2 * n
+        and (wgt is None or wgt.size == n)
+        and (mask is None or mask.size == n)
+        and info.size == N_INFO
+    )
+    if not ok:
+        raise ValueError("gridder array sizes disagree")
+    return int(fn(ctypes.byref(o), nrow, nchan, _ptr(uvw), _ptr(freq), _ptr(dirty), _ptr(wgt),
+                  wgt_kind, _ptr(mask), _ptr(vis), _ptr(info)))
+
+
+def wgridder_dirty2vis_f64(opts, uvw, freq, dirty, wgt, mask, vis, info):
+    """`rang_core_wgridder_dirty2vis_ex_f64` (core/py/src/lib.rs:1397-1420)."""
+    return _d2v(opts, uvw, freq, dirty, wgt, mask, vis, info, np.float64,
+                library().rang_core_wgridder_dirty2vis_ex_f64)
+
+
+def wgridder_dirty2vis_f32(opts, uvw, freq, dirty, wgt, mask, vis, info):
+    """`rang_core_wgridder_dirty2vis_ex_f32` (core/py/src/lib.rs:1422-1445)."""
+    return _d2v(opts, uvw, freq, dirty, wgt, mask, vis, info, np.float32,
+                library().rang_core_wgridder_dirty2vis_ex_f32)
+
+
+def wgridder_kernel_row(index, out):
+    """Row `index` of the kernel table into seven doubles; 1 past the end."""
+    out = _checked(out, np.float64, "out")
+    if out.size != 7:
+        return 1
+    return int(library().rang_core_wgridder_kernel_row(int(index), _ptr(out)))
+
+
+def wgridder_last_error():
+    """The message of this thread's last status-1 gridder failure."""
+    lib = library()
+    n = lib.rang_core_wgridder_last_error(None, 0)
+    buf = ctypes.create_string_buffer(max(int(n), 1))
+    lib.rang_core_wgridder_last_error(ctypes.cast(buf, ctypes.c_void_p), n)
+    return buf.raw[:n].decode("utf-8", "replace")

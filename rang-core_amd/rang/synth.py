@@ -1,0 +1,103 @@
+"""Seeded synthetic interferometer data for the gridder workloads.
+
+An array of antennas at uniform-random positions in a disk (local east,
+north), Earth-rotation synthesis of the baselines' uvw tracks (the rotation
+of examples/bench_df2.py:98-121 in rang-core), and Stokes-I visibilities of
+point sources by a direct sum, plus complex Gaussian noise. No reference code
+runs here; the recipe is the workload's description.
+"""
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+OMEGA_EARTH = 7.2921159e-5  # rad/s
+ARCSEC = np.pi / (180.0 * 3600.0)
+
+
+def antennas(nant, radius_m, seed):
+    """(nant, 3) east, north, up positions uniform in a disk."""
+    rng = np.random.default_rng(seed)
+    r = radius_m * np.sqrt(rng.uniform(0.0, 1.0, nant))
+    phi = rng.uniform(0.0, 2.0 * np.pi, nant)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), np.zeros(nant)], axis=1)
+
+
+def baselines_xyz(enu, lat_deg):
+    """Baseline vectors (i < j) in the equatorial frame (X to the meridian at
+    the equator, Z to the pole) from local east-north-up positions."""
+    lat = np.deg2rad(lat_deg)
+    i1, i2 = np.triu_indices(enu.shape[0], 1)
+    b = enu[i2] - enu[i1]
+    e, n, u = b[:, 0], b[:, 1], b[:, 2]
+    x = -np.sin(lat) * n + np.cos(lat) * u
+    y = e
+    z = np.cos(lat) * n + np.sin(lat) * u
+    return np.stack([x, y, z], axis=1)
+
+
+def uvw_tracks(bxyz, ntime, dt_s, dec_deg):
+    """(ntime * nbl, 3) uvw in metres, rows time-major, hour angles centred
+    on transit."""
+    t = (np.arange(ntime) - 0.5 * (ntime - 1)) * dt_s
+    ha = OMEGA_EARTH * t
+    dec = np.deg2rad(dec_deg)
+    sh, ch = np.sin(ha)[:, None], np.cos(ha)[:, None]
+    x, y, z = bxyz[None, :, 0], bxyz[None, :, 1], bxyz[None, :, 2]
+    u = sh * x + ch * y
+    v = -np.sin(dec) * ch * x + np.sin(dec) * sh * y + np.cos(dec) * z
+    w = np.cos(dec) * ch * x - np.cos(dec) * sh * y + np.sin(dec) * z
+    return np.stack([u, v, w], axis=-1).reshape(-1, 3)
+
+
+def point_sky(nsrc, radius_deg, flux_lo, flux_hi, seed):
+    """Fluxes log-uniform in [flux_lo, flux_hi], positions (l, m) uniform in a
+    disk of radius_deg."""
+    rng = np.random.default_rng(seed)
+    flux = np.exp(rng.uniform(np.log(flux_lo), np.log(flux_hi), nsrc))
+    r = np.deg2rad(radius_deg) * np.sqrt(rng.uniform(0.0, 1.0, nsrc))
+    phi = rng.uniform(0.0, 2.0 * np.pi, nsrc)
+    return flux, r * np.cos(phi), r * np.sin(phi)
+
+
+def visibilities(uvw, freq, sky, noise_jy, seed, dtype=np.complex64):
+    """V = sum_s F_s exp(-2 pi i (u l + v m + w (n-1))) f/c plus complex
+    Gaussian noise of noise_jy per component; (nrow, nchan)."""
+    flux, l, m = sky
+    f = np.asarray(freq) / C_LIGHT
+    vis = np.zeros((uvw.shape[0], f.size), np.complex128)
+    for s in range(flux.size):
+        n1 = np.sqrt(1.0 - l[s] ** 2 - m[s] ** 2) - 1.0
+        path = uvw[:, 0] * l[s] + uvw[:, 1] * m[s] + uvw[:, 2] * n1
+        vis += flux[s] * np.exp(-2j * np.pi * path[:, None] * f[None, :])
+    rng = np.random.default_rng(seed + 1)
+    vis += noise_jy * (rng.standard_normal(vis.shape) + 1j * rng.standard_normal(vis.shape))
+    return vis.astype(dtype)
+
+
+def imaging_case(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, *, nant=64, radius_m=8000.0,
+                 lat_deg=-30.7, dec_deg=-45.0, dt_s=8.0, nsrc=20, sky_radius_deg=0.5,
+                 flux=(1e-3, 1.0), noise_jy=1e-3, seed=1234, dtype=np.complex64):
+    enu = antennas(nant, radius_m, seed)
+    uvw = uvw_tracks(baselines_xyz(enu, lat_deg), ntime, dt_s, dec_deg)
+    freq = np.linspace(f_lo, f_hi, nchan)
+    sky = point_sky(nsrc, sky_radius_deg, flux[0], flux[1], seed + 7)
+    vis = visibilities(uvw, freq, sky, noise_jy, seed, dtype)
+    return {
+        "uvw": uvw,
+        "freq": freq,
+        "vis": vis,
+        "wgt": np.ones(vis.shape, np.float32 if dtype == np.complex64 else np.float64),
+        "npix": npix,
+        "pixsize": pix_arcsec * ARCSEC,
+        "sky": sky,
+    }
+
+
+def config1(seed=1234, ntime=100):
+    """64 antennas within 8 km at latitude -30.7 deg, 2016 baselines x 100
+    times of 8 s at dec -45 deg x 16 channels over 1.30-1.45 GHz (3,225,600
+    complex64 visibilities), 20 point sources, 1 mJy noise, unit weights,
+    1024^2 image at 2 arcsec, epsilon 1e-5."""
+    case = imaging_case(ntime, 16, 1.30e9, 1.45e9, 1024, 2.0, seed=seed)
+    case["epsilon"] = 1e-5
+    return case

@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every symbol its header declares, and
+the checks that need no GPU behave as the reference's (CPU only)."""
+
+import re
+
+import numpy as np
+import pytest
+from conftest import ROOT, ref_case
+
+from oracle import oracle as O
+from rang import _core, core_wgridder
+
+HEADER = ROOT / "include" / "rang_core_wgridder.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(rang_(?:core|amd)_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) == 20
+    lib = _core.library()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_core.SIGNATURES)
+
+
+def test_options_struct_layout():
+    # 9 size/double fields, 6 int32, 1 int64, 2 size_t: 120 bytes
+    import ctypes
+
+    assert ctypes.sizeof(_core.WgridderOptions) == 120
+    assert _core.WgridderOptions.kernel_index.offset == 96
+
+
+def test_kernel_rows_through_the_abi_are_the_table():
+    table = core_wgridder.kernel_table()
+    assert len(table) == 939 == O.kernel_count()
+    for i in range(0, 939, 7):
+        assert table[i] == O.kernel_row(i)
+    assert core_wgridder.kernel_row(939) is None
+
+
+def test_host_side_errors_match_the_reference():
+    uvw, freq, vis, wgt, nx, ny, dx = ref_case(nrow=6)
+    kw = dict(uvw=uvw, npix_x=nx, npix_y=ny, pixsize_x=dx, pixsize_y=dx, vis=vis, wgt=wgt)
+    with pytest.raises(ValueError, match="ascending"):
+        core_wgridder.vis2dirty(freq=freq[::-1], epsilon=1e-6, **kw)
+    with pytest.raises(ValueError, match="positive"):
+        core_wgridder.vis2dirty(freq=freq, epsilon=0.0, **kw)
+    with pytest.raises(ValueError, match="positive"):
+        core_wgridder.vis2dirty(freq=-freq, epsilon=1e-6, **kw)
+    with pytest.raises(ValueError, match="sizes disagree"):
+        _core.wgridder_vis2dirty_f64([nx, ny, dx, dx, 1e-6, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0],
+                                     uvw, freq, vis[:-1], wgt, None, None,
+                                     np.empty((nx, ny)), None, np.empty(14))
+    with pytest.raises(ValueError, match="C-contiguous"):
+        _core.wgridder_vis2dirty_f64([nx, ny, dx, dx, 1e-6, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0],
+                                     uvw, freq, np.asfortranarray(vis), wgt, None, None,
+                                     np.empty((nx, ny)), None, np.empty(14))
+    with pytest.raises(ValueError, match="two weights"):
+        _core.wgridder_vis2dirty_f64([nx, ny, dx, dx, 1e-6, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0],
+                                     uvw, freq, vis, wgt, wgt.astype(np.float32), None,
+                                     np.empty((nx, ny)), None, np.empty(14))
+    with pytest.raises(ValueError, match="18 gridder options"):
+        _core.wgridder_vis2dirty_f64([1.0] * 17, uvw, freq, vis, wgt, None, None,
+                                     np.empty((nx, ny)), None, np.empty(14))
