@@ -72,6 +72,10 @@ enum Slot : int {
     kSlotTmp,
     kSlotProj,
     kSlotImg,
+    kSlotOrder,
+    kSlotScanPart,
+    kSlotItems,
+    kSlotItemOff,
     kSlotCount
 };
 
@@ -81,6 +85,25 @@ struct Workspace {
     void* ptr[kSlotCount] = {};
     std::size_t size[kSlotCount] = {};
     std::mutex mu;
+    // Pinned host staging for the small transfers of a call.
+    void* pinned = nullptr;
+    std::size_t pinned_size = 0;
+    // Signature of the plan whose tables (twiddles, coefficients, screen and
+    // correction tables, tile order) the table slots hold, and their layout.
+    std::vector<double> table_key;
+    FftDesc du, dv;
+    std::vector<double> foc_key;
+    std::vector<int> order_key;
+    void* host(std::size_t bytes) {
+        if (pinned_size < bytes) {
+            if (pinned) HIPCHK(hipHostFree(pinned));
+            pinned = nullptr;
+            pinned_size = 0;
+            HIPCHK(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+            pinned_size = bytes;
+        }
+        return pinned;
+    }
     void* get(int slot, std::size_t bytes) {
         bytes = std::max<std::size_t>(bytes, 256);
         if (size[slot] < bytes) {
@@ -288,14 +311,20 @@ __device__ __forceinline__ cx<T> shift_phase(const Geo& g, const double b[3], do
 
 // ------------------------------------------------------------ scan
 
+constexpr int kScanBlocks = 1024;
+
+// Active count and |w| range per block (wgridder.rs:1051-1077), reduced by
+// k_scan_final; the index arithmetic stays in 32 bits (n < 2^31).
 template <class T, bool USE_VIS>
-__global__ __launch_bounds__(256) void k_scan(DevIn in, Geo g, unsigned long long* out) {
+__global__ __launch_bounds__(256) void k_scan(DevIn in, Geo g, unsigned long long* bcnt,
+                                              double* blo, double* bhi) {
     unsigned long long cnt = 0;
     double lo = 1e300, hi = -1e300;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
+    const unsigned n = unsigned(g.n), nchan = unsigned(g.nchan);
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         if (!active<T, USE_VIS>(in, i)) continue;
-        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
+        const unsigned row = i / nchan, ch = i - row * nchan;
         const double w = fabs(g.fw * in.uvw[3 * row + 2] * in.foc[ch]);
         ++cnt;
         lo = fmin(lo, w);
@@ -315,11 +344,41 @@ __global__ __launch_bounds__(256) void k_scan(DevIn in, Geo g, unsigned long lon
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && s_cnt[0] > 0) {
-        atomicAdd(&out[0], s_cnt[0]);
-        // |w| >= 0: the bit patterns order as the values
-        atomicMin(&out[1], (unsigned long long)__double_as_longlong(s_lo[0]));
-        atomicMax(&out[2], (unsigned long long)__double_as_longlong(s_hi[0]));
+    if (threadIdx.x == 0) {
+        bcnt[blockIdx.x] = s_cnt[0];
+        blo[blockIdx.x] = s_lo[0];
+        bhi[blockIdx.x] = s_hi[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_final(const unsigned long long* bcnt, const double* blo,
+                                                    const double* bhi, int nblk,
+                                                    unsigned long long* out) {
+    __shared__ unsigned long long s_cnt[256];
+    __shared__ double s_lo[256], s_hi[256];
+    unsigned long long c = 0;
+    double lo = 1e300, hi = -1e300;
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        c += bcnt[b];
+        lo = fmin(lo, blo[b]);
+        hi = fmax(hi, bhi[b]);
+    }
+    s_cnt[threadIdx.x] = c;
+    s_lo[threadIdx.x] = lo;
+    s_hi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+            s_lo[threadIdx.x] = fmin(s_lo[threadIdx.x], s_lo[threadIdx.x + h]);
+            s_hi[threadIdx.x] = fmax(s_hi[threadIdx.x], s_hi[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_cnt[0];
+        out[1] = (unsigned long long)__double_as_longlong(s_lo[0]);
+        out[2] = (unsigned long long)__double_as_longlong(s_hi[0]);
     }
 }
 
@@ -327,34 +386,42 @@ __global__ __launch_bounds__(256) void k_scan(DevIn in, Geo g, unsigned long lon
 
 // Bucket of an active visibility: (tile, spill class, first plane). The
 // spill class says whether its footprint reaches the next tile in u (2) and
-// in v (1).
+// in v (1). One thread per row walks its channels; a run of channels in the
+// same bucket (the usual case on short baselines) costs one atomic.
 template <class T, bool USE_VIS>
 __global__ __launch_bounds__(256) void k_keys(DevIn in, Geo g, std::uint32_t* keys,
                                               std::uint32_t* counts) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
-        if (!active<T, USE_VIS>(in, i)) {
-            keys[i] = kNoKey;
-            continue;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.nrow) return;
+    double c[3];
+    coord_of(in, g, row, c);
+    const double sgn = c[2] < 0.0 ? -1.0 : 1.0;
+    std::uint32_t run_key = kNoKey, run = 0;
+    const unsigned base = unsigned(row) * unsigned(g.nchan);
+    for (int ch = 0; ch < g.nchan; ++ch) {
+        const unsigned i = base + unsigned(ch);
+        std::uint32_t key = kNoKey;
+        if (active<T, USE_VIS>(in, i)) {
+            const double f = in.foc[ch];
+            const double u = c[0] * sgn * f, v = c[1] * sgn * f, w = c[2] * sgn * f;
+            double uf, vf;
+            long long iu0, iv0;
+            getpix(g, u, v, uf, vf, iu0, iv0);
+            const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
+            const int tu = su / g.Tu, tv = sv / g.Tv;
+            const int ou = su - tu * g.Tu, ov = sv - tv * g.Tv;
+            const int cat = ((ou + g.supp > g.Tu) ? 2 : 0) | ((ov + g.supp > g.Tv) ? 1 : 0);
+            key = std::uint32_t((((long long)tu * g.ntv + tv) * 4 + cat) * g.nb + minplane(g, w));
         }
-        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
-        double c[3];
-        coord_of(in, g, row, c);
-        const double sgn = c[2] < 0.0 ? -1.0 : 1.0;
-        const double f = in.foc[ch];
-        const double u = c[0] * sgn * f, v = c[1] * sgn * f, w = c[2] * sgn * f;
-        double uf, vf;
-        long long iu0, iv0;
-        getpix(g, u, v, uf, vf, iu0, iv0);
-        const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
-        const int tu = su / g.Tu, tv = sv / g.Tv;
-        const int ou = su - tu * g.Tu, ov = sv - tv * g.Tv;
-        const int cat = ((ou + g.supp > g.Tu) ? 2 : 0) | ((ov + g.supp > g.Tv) ? 1 : 0);
-        const std::uint32_t key = std::uint32_t((((long long)tu * g.ntv + tv) * 4 + cat) * g.nb +
-                                                minplane(g, w));
         keys[i] = key;
-        atomicAdd(&counts[key], 1u);
+        if (key != run_key) {
+            if (run_key != kNoKey) atomicAdd(&counts[run_key], run);
+            run_key = key;
+            run = 0;
+        }
+        ++run;
     }
+    if (run_key != kNoKey) atomicAdd(&counts[run_key], run);
 }
 
 constexpr int kScanItems = 4096;  // per block of 1024 threads
@@ -395,61 +462,74 @@ __global__ __launch_bounds__(256) void k_scan_add(std::uint32_t* out, const std:
 // One record per active visibility in bucket order. MODE 0: a dirty image
 // alone (the data value), 1: with a PSF (the unit value; the data value is
 // formed while gridding), 2: dirty2vis (weight times conjugate shift phase).
+// One thread per row; each run of channels in one bucket takes a contiguous
+// block of slots with one atomic, in channel order.
+template <class T, int MODE>
+__device__ __forceinline__ Rec<T> make_rec(const DevIn& in, const Geo& g, unsigned i, int ch,
+                                           const double c[3], std::uint32_t key) {
+    const double imflip = c[2] < 0.0 ? -1.0 : 1.0;  // FixW (wgridder.rs:583-586)
+    const double b[3] = {c[0] * imflip, c[1] * imflip, c[2] * imflip};
+    const double f = in.foc[ch];
+    const double u = b[0] * f, v = b[1] * f, w = b[2] * f;
+    double uf, vf;
+    long long iu0, iv0;
+    getpix(g, u, v, uf, vf, iu0, iv0);
+    const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
+    const int ou = su % g.Tu, ov = sv % g.Tv;
+    Rec<T> r;
+    r.w = w;
+    r.x0 = T(-uf * 2.0 + (double(g.supp) - 1.0));
+    r.y0 = T(-vf * 2.0 + (double(g.supp) - 1.0));
+    r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8) | (std::uint32_t(key % std::uint32_t(g.nb)) << 16);
+    r.idx = i | (imflip < 0.0 ? kFlip : 0u);
+    T wv = T(1);
+    const bool haswgt = wgt_get<T>(in, i, wv);
+    if constexpr (MODE == 2) {
+        // wgridder.rs:3146-3159
+        if (g.shifting) {
+            const cx<T> ph = shift_phase<T>(g, b, imflip, f, -1.0);
+            r.re = ph.re * wv;
+            r.im = ph.im * wv;
+        } else {
+            r.re = wv;
+            r.im = T(0);
+        }
+    } else {
+        // unit value (wgridder.rs:1414-1424)
+        cx<T> p = {T(1), T(0)};
+        if (g.shifting) p = cmul(p, shift_phase<T>(g, b, imflip, f, 1.0));
+        if (haswgt) p = {p.re * wv, p.im * wv};
+        p.im = p.im * T(imflip);
+        if constexpr (MODE == 0) {
+            // data value (wgridder.rs:1432-1435)
+            const cx<T> d = cmul(vis_get<T>(in, i), cx<T>{p.re, p.im * T(imflip)});
+            p = {d.re, d.im * T(imflip)};
+        }
+        r.re = p.re;
+        r.im = p.im;
+    }
+    return r;
+}
+
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_scatter(DevIn in, Geo g, const std::uint32_t* keys,
                                                  const std::uint32_t* offsets,
                                                  std::uint32_t* cursor, Rec<T>* recs) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
-        const std::uint32_t key = keys[i];
-        if (key == kNoKey) continue;
-        const std::uint32_t slot = offsets[key] + atomicAdd(&cursor[key], 1u);
-        const int row = int(i / g.nchan), ch = int(i - (long long)row * g.nchan);
-        double c[3];
-        coord_of(in, g, row, c);
-        const double imflip = c[2] < 0.0 ? -1.0 : 1.0;  // FixW (wgridder.rs:583-586)
-        const double b[3] = {c[0] * imflip, c[1] * imflip, c[2] * imflip};
-        const double f = in.foc[ch];
-        const double u = b[0] * f, v = b[1] * f, w = b[2] * f;
-        double uf, vf;
-        long long iu0, iv0;
-        getpix(g, u, v, uf, vf, iu0, iv0);
-        const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
-        const int ou = su % g.Tu, ov = sv % g.Tv;
-        Rec<T> r;
-        r.w = w;
-        r.x0 = T(-uf * 2.0 + (double(g.supp) - 1.0));
-        r.y0 = T(-vf * 2.0 + (double(g.supp) - 1.0));
-        r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8) |
-                (std::uint32_t(key % std::uint32_t(g.nb)) << 16);
-        r.idx = std::uint32_t(i) | (imflip < 0.0 ? kFlip : 0u);
-        T wv = T(1);
-        const bool haswgt = wgt_get<T>(in, i, wv);
-        if constexpr (MODE == 2) {
-            // wgridder.rs:3146-3159
-            if (g.shifting) {
-                const cx<T> ph = shift_phase<T>(g, b, imflip, f, -1.0);
-                r.re = ph.re * wv;
-                r.im = ph.im * wv;
-            } else {
-                r.re = wv;
-                r.im = T(0);
-            }
-        } else {
-            // unit value (wgridder.rs:1414-1424)
-            cx<T> p = {T(1), T(0)};
-            if (g.shifting) p = cmul(p, shift_phase<T>(g, b, imflip, f, 1.0));
-            if (haswgt) p = {p.re * wv, p.im * wv};
-            p.im = p.im * T(imflip);
-            if constexpr (MODE == 0) {
-                // data value (wgridder.rs:1432-1435)
-                const cx<T> d = cmul(vis_get<T>(in, i), cx<T>{p.re, p.im * T(imflip)});
-                p = {d.re, d.im * T(imflip)};
-            }
-            r.re = p.re;
-            r.im = p.im;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.nrow) return;
+    double c[3];
+    coord_of(in, g, row, c);
+    const unsigned base = unsigned(row) * unsigned(g.nchan);
+    int ch = 0;
+    while (ch < g.nchan) {
+        const std::uint32_t key = keys[base + ch];
+        int end = ch + 1;
+        while (end < g.nchan && keys[base + end] == key) ++end;
+        if (key != kNoKey) {
+            std::uint32_t slot = offsets[key] + atomicAdd(&cursor[key], std::uint32_t(end - ch));
+            for (int k = ch; k < end; ++k) recs[slot++] = make_rec<T, MODE>(in, g, base + k, k, c, key);
         }
-        recs[slot] = r;
+        ch = end;
     }
 }
 
@@ -525,94 +605,246 @@ __device__ __forceinline__ T w_factor(const Geo& g, const T* c, int nth, int p, 
 
 constexpr int kGridThreads = 256;
 
-// One workgroup per (tile, group of G planes). The accumulators of the
-// tile's cells on each plane of the group live in LDS; visibilities of this
-// tile and the spilling ones of the three preceding tiles add their taps that
-// fall inside it; the cells are written to the planes once. NOUT = 2 grids a
-// dirty image and a PSF together (wgridder.rs:2128-2152).
+// Work items of the gridding: per (tile rank, plane group) the visibilities
+// of the nine source ranges that reach the tile's cells on the group's
+// planes (see k_grid), cut into chunks of at most `rc`. Ranks follow
+// `order` (densest tiles first), so the longest items start first.
+__device__ __forceinline__ void source_of(int src, int& du, int& dv, int& cat) {
+    // own tile with every spill class, the tile before in u with the classes
+    // spilling in u, the one before in v with those spilling in v, and the
+    // diagonal one spilling in both
+    if (src < 4) {
+        du = 0, dv = 0, cat = src;
+    } else if (src < 6) {
+        du = 1, dv = 0, cat = 2 + (src - 4);
+    } else if (src < 8) {
+        du = 0, dv = 1, cat = 1 + 2 * (src - 6);
+    } else {
+        du = 1, dv = 1, cat = 3;
+    }
+}
+
+__device__ __forceinline__ void source_range(const Geo& g, const std::uint32_t* off, int tu, int tv,
+                                             int src, int mlo, int mhi, std::uint32_t& a,
+                                             std::uint32_t& b) {
+    int du, dv, cat;
+    source_of(src, du, dv, cat);
+    const int stile = ((tu - du + g.ntu) % g.ntu) * g.ntv + (tv - dv + g.ntv) % g.ntv;
+    const long long k0 = ((long long)stile * 4 + cat) * g.nb;
+    a = off[k0 + mlo];
+    b = off[k0 + mhi + 1];
+}
+
+__global__ __launch_bounds__(256) void k_grid_items(Geo g, const std::uint32_t* __restrict__ off,
+                                                    const std::uint32_t* __restrict__ order, int G,
+                                                    int ngroups, int S, std::uint32_t rc,
+                                                    std::uint32_t* __restrict__ nitems) {
+    const int ntg = g.ntu * g.ntv * ngroups;
+    const int tg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tg > ntg) return;
+    if (tg == ntg) {
+        nitems[ntg] = 0;
+        return;
+    }
+    const int rank = tg / ngroups, grp = tg - rank * ngroups;
+    const int tile = int(order[rank]);
+    const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
+    const int P = g.wgrid ? g.nplanes : 1;
+    const int p0 = grp * G, p1 = min(P, p0 + G);
+    const int mlo = g.wgrid ? max(0, p0 - S + 1) : 0, mhi = g.wgrid ? min(g.nb - 1, p1 - 1) : 0;
+    std::uint32_t total = 0;
+    for (int src = 0; src < 9; ++src) {
+        std::uint32_t a, b;
+        source_range(g, off, tu, tv, src, mlo, mhi, a, b);
+        total += b - a;
+    }
+    nitems[tg] = max(1u, (total + rc - 1) / rc);
+}
+
+// Cells of the (tile, plane group)s gridded by more than one item, zeroed
+// before those items add into them.
+template <class T, int NOUT>
+__global__ __launch_bounds__(256) void k_zero_split(Geo g, const std::uint32_t* __restrict__ item_off,
+                                                    const std::uint32_t* __restrict__ order, int G,
+                                                    int ngroups, cx<T>* __restrict__ grid) {
+    const int tg = blockIdx.x;
+    if (item_off[tg + 1] - item_off[tg] <= 1) return;
+    const int rank = tg / ngroups, grp = tg - rank * ngroups;
+    const int tile = int(order[rank]);
+    const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
+    const int P = g.wgrid ? g.nplanes : 1;
+    const int p0 = grp * G, np = min(P, p0 + G) - p0;
+    const int cells = g.Tu * g.Tv;
+    const long long plane = (long long)g.nu * g.nv;
+    for (int e = threadIdx.x; e < NOUT * np * cells; e += blockDim.x) {
+        const int o = e / (np * cells), rem0 = e - o * np * cells;
+        const int pl = rem0 / cells, rem = rem0 - pl * cells;
+        const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
+        const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
+        const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
+        grid[(long long)o * P * plane + (long long)(p0 + pl) * plane + (long long)grow * g.nv + gcol] =
+            cx<T>{T(0), T(0)};
+    }
+}
+
+// The item `item` of a prefix array of per-unit item counts: the unit and
+// the chunk within it, or false past the last item.
+__device__ __forceinline__ bool find_item(const std::uint32_t* item_off, int nunits, std::uint32_t item,
+                                          int& unit, std::uint32_t& chunk, std::uint32_t& nchunks) {
+    if (item >= item_off[nunits]) return false;
+    int lo = 0, hi = nunits;  // item_off[lo] <= item < item_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (item_off[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    unit = lo;
+    chunk = item - item_off[lo];
+    nchunks = item_off[lo + 1] - item_off[lo];
+    return true;
+}
+
+constexpr int kGridWaves = kGridThreads / 64;
+
+// Gridding of one item: a chunk of the visibilities reaching one tile on
+// one group of planes (see k_grid_items), accumulated in LDS and added to the
+// planes (written when the tile has one item, added with global atomics into
+// zeroed cells otherwise). One wave takes one visibility at a time (RPW of
+// them for supports up to 5), one lane per tap: the lanes of an LDS atomic
+// always address distinct cells, however many visibilities share a position
+// (the channels and times of a short baseline do). NOUT = 2 grids a dirty
+// image and a PSF together (wgridder.rs:2128-2152). Row stride TvP is padded
+// so that the S x S footprint spreads over the banks.
 template <class T, int S, int NOUT>
 __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
                                                        const std::uint32_t* __restrict__ off,
                                                        const T* __restrict__ coeff, DevIn in,
-                                                       int G, cx<T>* __restrict__ grid) {
+                                                       int G, int ngroups, int TvP, std::uint32_t rc,
+                                                       const std::uint32_t* __restrict__ item_off,
+                                                       const std::uint32_t* __restrict__ order,
+                                                       cx<T>* __restrict__ grid) {
     constexpr int NE = ne_of<S>();
+    constexpr int SS = S * S;
+    constexpr int RPW = SS <= 16 ? 4 : SS <= 32 ? 2 : 1;
+    constexpr int LPG = 64 / RPW;
+    constexpr int CPL = (SS + LPG - 1) / LPG;
     __shared__ T coef[2 * NE * S];
+    __shared__ std::uint32_t s_start[9], s_pre[10];
+    __shared__ int s_dou[9], s_dov[9];
     extern __shared__ unsigned char smem[];
     T* acc = reinterpret_cast<T*>(smem);
-    const int cells = g.Tu * g.Tv;
-    const int tile = blockIdx.x;
-    const int tu = tile / g.ntv, tv = tile - (tile / g.ntv) * g.ntv;
-    const int p0 = int(blockIdx.y) * G;
+    const int ntg = g.ntu * g.ntv * ngroups;
+    int tg;
+    std::uint32_t chunk, nchunks;
+    if (!find_item(item_off, ntg, blockIdx.x, tg, chunk, nchunks)) return;
+    const int rank = tg / ngroups, grp = tg - rank * ngroups;
+    const int tile = int(order[rank]);
+    const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
     const int P = g.wgrid ? g.nplanes : 1;
-    const int p1 = min(P, p0 + G);
-    const int np = p1 - p0;
-    for (int e = threadIdx.x; e < NOUT * np * 2 * cells; e += kGridThreads) acc[e] = T(0);
+    const int p0 = grp * G, p1 = min(P, p0 + G), np = p1 - p0;
+    const int pstride = g.Tu * TvP;  // one plane of one output, one of re / im
+    for (int e = threadIdx.x; e < NOUT * np * 2 * pstride; e += kGridThreads) acc[e] = T(0);
     for (int e = threadIdx.x; e < 2 * NE * S; e += kGridThreads) coef[e] = coeff[e];
-    __syncthreads();
     const int mlo = g.wgrid ? max(0, p0 - S + 1) : 0;
     const int mhi = g.wgrid ? min(g.nb - 1, p1 - 1) : 0;
-    for (int src = 0; src < 9; ++src) {
-        // (du, dv, class): own tile with every class, the tile before in u
-        // with classes spilling in u, before in v spilling in v, and the
-        // diagonal one spilling in both.
+    if (threadIdx.x < 9) {
+        const int src = threadIdx.x;
         int du, dv, cat;
-        if (src < 4) {
-            du = 0, dv = 0, cat = src;
-        } else if (src < 6) {
-            du = 1, dv = 0, cat = 2 + (src - 4);
-        } else if (src < 8) {
-            du = 0, dv = 1, cat = 1 + 2 * (src - 6);
-        } else {
-            du = 1, dv = 1, cat = 3;
-        }
-        const int stile = ((tu - du + g.ntu) % g.ntu) * g.ntv + (tv - dv + g.ntv) % g.ntv;
-        const long long k0 = ((long long)stile * 4 + cat) * g.nb;
-        const std::uint32_t a = off[k0 + mlo], b = off[k0 + mhi + 1];
-        const int dou = du ? -g.Tu : 0, dov = dv ? -g.Tv : 0;
-        for (std::uint32_t r = a + threadIdx.x; r < b; r += kGridThreads) {
-            const Rec<T> rc = recs[r];
-            const int ou = int(rc.pos & 255u) + dou, ov = int((rc.pos >> 8) & 255u) + dov;
-            const int mp = int(rc.pos >> 16);
-            T ku[S], kv[S];
-            eval_taps2<T, S>(coef, rc.x0, rc.y0, ku, kv);
-            cx<T> val[NOUT];
-            if constexpr (NOUT == 2) {
-                const T fl = (rc.idx & kFlip) ? T(-1) : T(1);
-                const cx<T> d = cmul(vis_get<T>(in, rc.idx & ~kFlip), cx<T>{rc.re, rc.im * fl});
-                val[0] = {d.re, d.im * fl};
-                val[1] = {rc.re, rc.im};
-            } else {
-                val[0] = {rc.re, rc.im};
-            }
-            const int pa = g.wgrid ? max(p0, mp) : 0;
-            const int pb = g.wgrid ? min(p1, mp + S) : 1;
-            for (int p = pa; p < pb; ++p) {
-                T kw = T(1);
-                if (g.wgrid) kw = w_factor<T, S>(g, coef, p - mp, p, rc.w);
+        source_of(src, du, dv, cat);
+        std::uint32_t a, b;
+        source_range(g, off, tu, tv, src, mlo, mhi, a, b);
+        s_start[src] = a;
+        s_pre[src + 1] = b - a;
+        s_dou[src] = du ? -g.Tu : 0;
+        s_dov[src] = dv ? -g.Tv : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_pre[0] = 0;
+        for (int k = 1; k <= 9; ++k) s_pre[k] += s_pre[k - 1];
+    }
+    __syncthreads();
+    const std::uint32_t jbeg = chunk * rc, jend = min(s_pre[9], jbeg + rc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int group = lane / LPG, local = lane - group * LPG;
+    int cu[CPL], cv[CPL];
 #pragma unroll
-                for (int cu = 0; cu < S; ++cu) {
-                    const int iu = ou + cu;
-                    if (unsigned(iu) >= unsigned(g.Tu)) continue;
-                    const T kuc = g.wgrid ? ku[cu] * kw : ku[cu];
+    for (int k = 0; k < CPL; ++k) {
+        const int q = min(local + k * LPG, SS - 1);
+        cu[k] = q / S;
+        cv[k] = q - (q / S) * S;
+    }
+    const int srclane = group * LPG;  // lane of the group holding w tap `local`
+    int src = 0;
+    const std::uint32_t step = kGridWaves * RPW;
+    for (std::uint32_t jw = jbeg + wave * RPW; jw < jend; jw += step) {
+        const std::uint32_t j = jw + group;
+        const bool valid = j < jend;
+        const std::uint32_t jj = valid ? j : jw;  // a valid record for every lane
+        while (jj >= s_pre[src + 1]) ++src;
+        const Rec<T> rc_ = recs[s_start[src] + (jj - s_pre[src])];
+        const int ou = int(rc_.pos & 255u) + s_dou[src], ov = int((rc_.pos >> 8) & 255u) + s_dov[src];
+        const int mp = int(rc_.pos >> 16);
+        cx<T> val[NOUT];
+        if constexpr (NOUT == 2) {
+            const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
+            const cx<T> d = cmul(vis_get<T>(in, rc_.idx & ~kFlip), cx<T>{rc_.re, rc_.im * fl});
+            val[0] = {d.re, d.im * fl};
+            val[1] = {rc_.re, rc_.im};
+        } else {
+            val[0] = {rc_.re, rc_.im};
+        }
+        T kuv[CPL];
+        int at[CPL];
+        bool inside[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            kuv[k] = eval_tap<T, S>(coef, cu[k], rc_.x0) * eval_tap<T, S>(coef, cv[k], rc_.y0);
+            const int iu = ou + cu[k], iv = ov + cv[k];
+            inside[k] = valid && local + k * LPG < SS && unsigned(iu) < unsigned(g.Tu) &&
+                        unsigned(iv) < unsigned(g.Tv);
+            at[k] = iu * TvP + iv;
+        }
+        if (g.wgrid) {
+            // lane `local` < S evaluates the w tap `local` (plane mp + local)
+            const int nth = min(local, S - 1);
+            const T kwv = w_factor<T, S>(g, coef, nth, mp + nth, rc_.w);
+#pragma unroll
+            for (int nth2 = 0; nth2 < S; ++nth2) {
+                const T kw = __shfl(kwv, srclane + nth2, 64);
+                const int p = mp + nth2;
+                if (p < p0 || p >= p1) continue;
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) {
+                    if (!inside[k]) continue;
+                    // (ku kw) kv as the reference forms ku * kw first
+                    const T f = kuv[k] * kw;
 #pragma unroll
                     for (int o = 0; o < NOUT; ++o) {
-                        T* are = acc + ((o * np + (p - p0)) * 2) * cells + iu * g.Tv;
-                        T* aim = are + cells;
-                        const T tr = val[o].re * kuc, ti = val[o].im * kuc;
-#pragma unroll
-                        for (int cv = 0; cv < S; ++cv) {
-                            const int iv = ov + cv;
-                            if (unsigned(iv) >= unsigned(g.Tv)) continue;
-                            atomicAdd(are + iv, tr * kv[cv]);
-                            atomicAdd(aim + iv, ti * kv[cv]);
-                        }
+                        T* are = acc + ((o * np + (p - p0)) * 2) * pstride + at[k];
+                        atomicAdd(are, val[o].re * f);
+                        atomicAdd(are + pstride, val[o].im * f);
                     }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                if (!inside[k]) continue;
+#pragma unroll
+                for (int o = 0; o < NOUT; ++o) {
+                    T* are = acc + (o * 2) * pstride + at[k];
+                    atomicAdd(are, val[o].re * kuv[k]);
+                    atomicAdd(are + pstride, val[o].im * kuv[k]);
                 }
             }
         }
     }
     __syncthreads();
     const long long plane = (long long)g.nu * g.nv;
+    const int cells = g.Tu * g.Tv;
     for (int e = threadIdx.x; e < np * cells; e += kGridThreads) {
         const int pl = e / cells, rem = e - pl * cells;
         const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
@@ -620,20 +852,25 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
         const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
 #pragma unroll
         for (int o = 0; o < NOUT; ++o) {
-            const T* are = acc + ((o * np + pl) * 2) * cells;
-            grid[(long long)o * P * plane + (long long)(p0 + pl) * plane + (long long)grow * g.nv + gcol] =
-                cx<T>{are[rem], are[cells + rem]};
+            const T* are = acc + ((o * np + pl) * 2) * pstride + i * TvP + j;
+            cx<T>* dst = grid + (long long)o * P * plane + (long long)(p0 + pl) * plane +
+                         (long long)grow * g.nv + gcol;
+            if (nchunks == 1) {
+                *dst = cx<T>{are[0], are[pstride]};
+            } else {
+                atomicAdd(&dst->re, are[0]);
+                atomicAdd(&dst->im, are[pstride]);
+            }
         }
     }
 }
 
 // ------------------------------------------------------------ transforms
 
-constexpr int kFftThreads = 256;
-
+// Largest transform held in LDS: 128 KiB of complex values.
 template <class T>
 constexpr int fft_maxtot() {
-    return sizeof(T) == 4 ? 8192 : 4096;  // 64 KiB of complex values
+    return sizeof(T) == 4 ? 16384 : 8192;
 }
 
 // Per-pixel tables (wgridder.rs:806-861): n-1+nshift for the w-screens and
@@ -699,8 +936,8 @@ __device__ __forceinline__ cx<T> screen_at(double w, double s, bool adjoint) {
 
 // v2d: inverse transform along v of one grid row, keeping the image columns
 // (wgridder.rs:2331-2338, 1745-1749).
-template <class T>
-__global__ __launch_bounds__(kFftThreads) void k_rows_inv(const cx<T>* __restrict__ grid,
+template <class T, int NT, int MAXTOT>
+__global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
                                                           const cx<T>* __restrict__ tw, int ny) {
     extern __shared__ unsigned char smem[];
@@ -708,17 +945,17 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_inv(const cx<T>* __restric
     const int nv = d.n;
     const long long row = blockIdx.x;
     const cx<T>* src = grid + row * nv;
-    for (int e = threadIdx.x; e < nv; e += kFftThreads) buf[e] = src[e];
+    for (int e = threadIdx.x; e < nv; e += NT) buf[e] = src[e];
     __syncthreads();
-    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, 1, tw, +1);
-    for (int j = threadIdx.x; j < ny; j += kFftThreads) tmp[row * ny + j] = buf[(nv - ny / 2 + j) % nv];
+    fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, +1);
+    for (int j = threadIdx.x; j < ny; j += NT) tmp[row * ny + j] = buf[(nv - ny / 2 + j) % nv];
 }
 
 // v2d: inverse transform along u of C image columns of one plane, then the
 // projection Re(g screen) of the image rows (wgridder.rs:2342-2397), one term
 // per plane and pixel.
-template <class T>
-__global__ __launch_bounds__(kFftThreads) void k_cols_inv(const cx<T>* __restrict__ tmp,
+template <class T, int NT, int MAXTOT>
+__global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
                                                           T* __restrict__ proj, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
                                                           const double* __restrict__ screen, int C) {
@@ -728,15 +965,15 @@ __global__ __launch_bounds__(kFftThreads) void k_cols_inv(const cx<T>* __restric
     const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
     const int p = blockIdx.y;
     const cx<T>* src = tmp + (long long)p * nu * ny;
-    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) {
+    for (int e = threadIdx.x; e < nu * nc; e += NT) {
         const int u = e / nc, c = e - u * nc;
         buf[c * nu + u] = src[(long long)u * ny + j0 + c];
     }
     __syncthreads();
-    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, nc, tw, +1);
+    fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, +1);
     const double w0 = g.wmin + double(p) * g.dw;
     T* dst = proj + (long long)p * nx * ny;
-    for (int e = threadIdx.x; e < nx * nc; e += kFftThreads) {
+    for (int e = threadIdx.x; e < nx * nc; e += NT) {
         const int i = e / nc, c = e - i * nc, j = j0 + c;
         const cx<T> gg = buf[c * nu + (nu - nx / 2 + i) % nu];
         T term;
@@ -791,8 +1028,8 @@ __global__ __launch_bounds__(256) void k_correct(const T* __restrict__ dirty, T*
 
 // d2v: C image columns times the plane's w-screen placed on the grid rows,
 // forward transform along u (wgridder.rs:2931-2967).
-template <class T>
-__global__ __launch_bounds__(kFftThreads) void k_cols_fwd(const T* __restrict__ img,
+template <class T, int NT, int MAXTOT>
+__global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
                                                           const double* __restrict__ screen, int C) {
@@ -801,10 +1038,10 @@ __global__ __launch_bounds__(kFftThreads) void k_cols_fwd(const T* __restrict__ 
     const int nu = d.n, ny = g.ny, nx = g.nx;
     const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
     const int p = blockIdx.y;
-    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) buf[e] = cx<T>{T(0), T(0)};
+    for (int e = threadIdx.x; e < nu * nc; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
     const double w0 = g.wmin + double(p) * g.dw;
-    for (int e = threadIdx.x; e < nx * nc; e += kFftThreads) {
+    for (int e = threadIdx.x; e < nx * nc; e += NT) {
         const int i = e / nc, c = e - i * nc, j = j0 + c;
         const T v = img[(long long)i * ny + j];
         cx<T> val;
@@ -817,9 +1054,9 @@ __global__ __launch_bounds__(kFftThreads) void k_cols_fwd(const T* __restrict__ 
         buf[c * nu + (nu - nx / 2 + i) % nu] = val;
     }
     __syncthreads();
-    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, nc, tw, -1);
+    fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, -1);
     cx<T>* dst = tmp + (long long)p * nu * ny;
-    for (int e = threadIdx.x; e < nu * nc; e += kFftThreads) {
+    for (int e = threadIdx.x; e < nu * nc; e += NT) {
         const int u = e / nc, c = e - u * nc;
         dst[(long long)u * ny + j0 + c] = buf[c * nu + u];
     }
@@ -827,70 +1064,124 @@ __global__ __launch_bounds__(kFftThreads) void k_cols_fwd(const T* __restrict__ 
 
 // d2v: one grid row from its image columns, forward transform along v
 // (wgridder.rs:2922-2929, 2968-2991).
-template <class T>
-__global__ __launch_bounds__(kFftThreads) void k_rows_fwd(const cx<T>* __restrict__ tmp,
+template <class T, int NT, int MAXTOT>
+__global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
                                                           cx<T>* __restrict__ grid, FftDesc d,
                                                           const cx<T>* __restrict__ tw, int ny) {
     extern __shared__ unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nv = d.n;
     const long long row = blockIdx.x;
-    for (int e = threadIdx.x; e < nv; e += kFftThreads) buf[e] = cx<T>{T(0), T(0)};
+    for (int e = threadIdx.x; e < nv; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
-    for (int j = threadIdx.x; j < ny; j += kFftThreads) buf[(nv - ny / 2 + j) % nv] = tmp[row * ny + j];
+    for (int j = threadIdx.x; j < ny; j += NT) buf[(nv - ny / 2 + j) % nv] = tmp[row * ny + j];
     __syncthreads();
-    fft_lds<T, kFftThreads, fft_maxtot<T>()>(buf, d, 1, tw, -1);
+    fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, -1);
     cx<T>* dst = grid + row * nv;
-    for (int e = threadIdx.x; e < nv; e += kFftThreads) dst[e] = buf[e];
+    for (int e = threadIdx.x; e < nv; e += NT) dst[e] = buf[e];
 }
 
-// d2v: degridding (wgridder.rs:2793-2901): per visibility of the tile, the
-// S x S taps of each of its planes against the plane grid, the planes added
-// in order, then the weight and shift phase (wgridder.rs:3146-3159).
-template <class T, int S>
+// Work items of the degridding: per tile rank (`order`, densest first) its
+// visibilities in chunks of at most `rc` (none for an empty tile).
+__global__ __launch_bounds__(256) void k_degrid_items(Geo g, const std::uint32_t* __restrict__ off,
+                                                      const std::uint32_t* __restrict__ order,
+                                                      std::uint32_t rc, std::uint32_t* __restrict__ nitems) {
+    const int nt = g.ntu * g.ntv;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nt) return;
+    if (r == nt) {
+        nitems[nt] = 0;
+        return;
+    }
+    const int tile = int(order[r]);
+    const std::uint32_t cnt = off[(long long)(tile + 1) * 4 * g.nb] - off[(long long)tile * 4 * g.nb];
+    nitems[r] = (cnt + rc - 1) / rc;
+}
+
+// d2v: degridding (wgridder.rs:2793-2901) of one item: per visibility of the
+// chunk, the S x S taps of each of its planes against the plane grid, the
+// planes added in order, then the weight and shift phase
+// (wgridder.rs:3146-3159). With STAGE the tile and its halo of every plane
+// are first copied to LDS (P (Tu+S-1) (Tv+S-1) values); otherwise the taps
+// read the planes in global memory. Each thread takes a contiguous part of
+// the chunk.
+template <class T, int S, bool STAGE>
 __global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict__ recs,
                                                 const std::uint32_t* __restrict__ off,
                                                 const T* __restrict__ coeff,
-                                                const cx<T>* __restrict__ grid,
+                                                const cx<T>* __restrict__ grid, std::uint32_t rc,
+                                                const std::uint32_t* __restrict__ item_off,
+                                                const std::uint32_t* __restrict__ order,
                                                 cx<T>* __restrict__ vis) {
     constexpr int NE = ne_of<S>();
     __shared__ T coef[2 * NE * S];
+    extern __shared__ unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    int rank;
+    std::uint32_t chunk, nchunks;
+    if (!find_item(item_off, g.ntu * g.ntv, blockIdx.x, rank, chunk, nchunks)) return;
+    const int tile = int(order[rank]);
+    const std::uint32_t t0 = off[(long long)tile * 4 * g.nb], t1 = off[(long long)(tile + 1) * 4 * g.nb];
+    const std::uint32_t a = t0 + chunk * rc, b = min(t1, a + rc);
     for (int e = threadIdx.x; e < 2 * NE * S; e += blockDim.x) coef[e] = coeff[e];
-    __syncthreads();
-    const int tile = blockIdx.x;
     const int tu = tile / g.ntv, tv = tile - (tile / g.ntv) * g.ntv;
-    const std::uint32_t a = off[(long long)tile * 4 * g.nb], b = off[(long long)(tile + 1) * 4 * g.nb];
     const int P = g.wgrid ? g.nplanes : 1;
     const long long plane = (long long)g.nu * g.nv;
-    for (std::uint32_t r = a + threadIdx.x; r < b; r += blockDim.x) {
-        const Rec<T> rc = recs[r];
-        const int ou = int(rc.pos & 255u), ov = int((rc.pos >> 8) & 255u), mp = int(rc.pos >> 16);
-        T ku[S], kv[S];
-        eval_taps2<T, S>(coef, rc.x0, rc.y0, ku, kv);
-        int rows[S], cols[S];
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            rows[k] = (tu * g.Tu + ou + k + g.nu / 2) % g.nu;
-            cols[k] = (tv * g.Tv + ov + k + g.nv / 2) % g.nv;
+    const int SU = g.Tu + S - 1, SV = g.Tv + S - 1;
+    if constexpr (STAGE) {
+        const int area = SU * SV;
+        for (int e = threadIdx.x; e < P * area; e += blockDim.x) {
+            const int p = e / area, rem = e - p * area;
+            const int i = rem / SV, j = rem - (rem / SV) * SV;
+            const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
+            const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
+            buf[e] = grid[(long long)p * plane + (long long)grow * g.nv + gcol];
         }
-        const T fl = (rc.idx & kFlip) ? T(-1) : T(1);
+    }
+    __syncthreads();
+    const std::uint32_t total = b - a;
+    const std::uint32_t part = (total + blockDim.x - 1) / blockDim.x;
+    const std::uint32_t j0 = a + threadIdx.x * part, j1 = min(b, j0 + part);
+    for (std::uint32_t r = j0; r < j1; ++r) {
+        const Rec<T> rc_ = recs[r];
+        const int ou = int(rc_.pos & 255u), ov = int((rc_.pos >> 8) & 255u), mp = int(rc_.pos >> 16);
+        T ku[S], kv[S];
+        eval_taps2<T, S>(coef, rc_.x0, rc_.y0, ku, kv);
+        int rows[S], cols[S];
+        if constexpr (!STAGE) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                rows[k] = (tu * g.Tu + ou + k + g.nu / 2) % g.nu;
+                cols[k] = (tv * g.Tv + ov + k + g.nv / 2) % g.nv;
+            }
+        }
+        const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
         cx<T> out = {T(0), T(0)};
         const int pa = g.wgrid ? mp : 0, pb = g.wgrid ? min(P, mp + S) : 1;
         for (int p = pa; p < pb; ++p) {
             T kw = T(1);
-            if (g.wgrid) kw = w_factor<T, S>(g, coef, p - mp, p, rc.w);
-            const cx<T>* gp = grid + (long long)p * plane;
+            if (g.wgrid) kw = w_factor<T, S>(g, coef, p - mp, p, rc_.w);
             T rr = T(0), ri = T(0);
 #pragma unroll
             for (int cu = 0; cu < S; ++cu) {
                 const T kuc = g.wgrid ? ku[cu] * kw : ku[cu];
-                const cx<T>* line = gp + (long long)rows[cu] * g.nv;
                 T tr = T(0), ti = T(0);
+                if constexpr (STAGE) {
+                    const cx<T>* line = buf + ((long long)p * SU + ou + cu) * SV + ov;
 #pragma unroll
-                for (int cv = 0; cv < S; ++cv) {
-                    const cx<T> v = line[cols[cv]];
-                    tr = fma(kv[cv], v.re, tr);
-                    ti = fma(kv[cv], v.im, ti);
+                    for (int cv = 0; cv < S; ++cv) {
+                        const cx<T> v = line[cv];
+                        tr = fma(kv[cv], v.re, tr);
+                        ti = fma(kv[cv], v.im, ti);
+                    }
+                } else {
+                    const cx<T>* line = grid + (long long)p * plane + (long long)rows[cu] * g.nv;
+#pragma unroll
+                    for (int cv = 0; cv < S; ++cv) {
+                        const cx<T> v = line[cols[cv]];
+                        tr = fma(kv[cv], v.re, tr);
+                        ti = fma(kv[cv], v.im, ti);
+                    }
                 }
                 rr = fma(kuc, tr, rr);
                 ri = fma(kuc, ti, ri);
@@ -899,10 +1190,10 @@ __global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict_
         }
         cx<T> res;
         if (g.shifting)
-            res = cmul(out, cx<T>{rc.re, rc.im});
+            res = cmul(out, cx<T>{rc_.re, rc_.im});
         else
-            res = {out.re * rc.re, out.im * rc.re};
-        vis[rc.idx & ~kFlip] = res;
+            res = {out.re * rc_.re, out.im * rc_.re};
+        vis[rc_.idx & ~kFlip] = res;
     }
 }
 
@@ -914,18 +1205,21 @@ constexpr int max_support() {
 }
 
 template <class T, int NOUT>
-void launch_grid(int S, dim3 grid, std::size_t lds, hipStream_t st, const Geo& g,
+void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
                  const Rec<T>* recs, const std::uint32_t* off, const T* coeff, const DevIn& in,
-                 int G, cx<T>* out) {
-#define GRID_CASE(SS)                                                                        \
-    case SS:                                                                                 \
-        if constexpr (SS <= max_support<T>()) {                                              \
-            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT>,                     \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))); \
-            k_grid<T, SS, NOUT><<<grid, kGridThreads, lds, st>>>(g, recs, off, coeff, in, G, out); \
-        } else {                                                                             \
-            throw DevError{"kernel support too large for this precision"};                   \
-        }                                                                                    \
+                 int G, int ngroups, int TvP, std::uint32_t rc, const std::uint32_t* item_off,
+                 const std::uint32_t* order, cx<T>* out) {
+#define GRID_CASE(SS)                                                                           \
+    case SS:                                                                                    \
+        if constexpr (SS <= max_support<T>()) {                                                 \
+            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT>,                        \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
+            k_grid<T, SS, NOUT><<<nblocks, kGridThreads, lds, st>>>(g, recs, off, coeff, in, G,  \
+                                                                     ngroups, TvP, rc, item_off, \
+                                                                     order, out);               \
+        } else {                                                                                \
+            throw DevError{"kernel support too large for this precision"};                      \
+        }                                                                                       \
         break;
     switch (S) {
         GRID_CASE(4) GRID_CASE(5) GRID_CASE(6) GRID_CASE(7) GRID_CASE(8) GRID_CASE(9)
@@ -938,15 +1232,22 @@ void launch_grid(int S, dim3 grid, std::size_t lds, hipStream_t st, const Geo& g
     HIPCHK(hipGetLastError());
 }
 
-template <class T>
-void launch_degrid(int S, int ntiles, hipStream_t st, const Geo& g, const Rec<T>* recs,
-                   const std::uint32_t* off, const T* coeff, const cx<T>* grid, cx<T>* vis) {
-#define DEGRID_CASE(SS)                                                                   \
-    case SS:                                                                              \
-        if constexpr (SS <= max_support<T>())                                             \
-            k_degrid<T, SS><<<ntiles, 256, 0, st>>>(g, recs, off, coeff, grid, vis);      \
-        else                                                                              \
-            throw DevError{"kernel support too large for this precision"};                \
+template <class T, bool STAGE>
+void launch_degrid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
+                   const Rec<T>* recs, const std::uint32_t* off, const T* coeff,
+                   const cx<T>* grid, std::uint32_t rc, const std::uint32_t* item_off,
+                   const std::uint32_t* order, cx<T>* vis) {
+#define DEGRID_CASE(SS)                                                                        \
+    case SS:                                                                                   \
+        if constexpr (SS <= max_support<T>()) {                                                \
+            if (STAGE)                                                                         \
+                HIPCHK(hipFuncSetAttribute((const void*)k_degrid<T, SS, STAGE>,                \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))); \
+            k_degrid<T, SS, STAGE><<<nblocks, 256, lds, st>>>(g, recs, off, coeff, grid, rc,   \
+                                                              item_off, order, vis);           \
+        } else {                                                                               \
+            throw DevError{"kernel support too large for this precision"};                     \
+        }                                                                                      \
         break;
     switch (S) {
         DEGRID_CASE(4) DEGRID_CASE(5) DEGRID_CASE(6) DEGRID_CASE(7) DEGRID_CASE(8) DEGRID_CASE(9)
@@ -960,18 +1261,17 @@ void launch_degrid(int S, int ntiles, hipStream_t st, const Geo& g, const Rec<T>
 }
 
 // Tile side along a grid axis of n cells: a divisor of n (so that no tile
-// is partial) of at least the support and at most `hi`, nearest 32.
-int choose_tile(int n, int supp, int hi) {
+// is partial) of at least the support and at most `hi`, nearest `target`.
+int choose_tile(int n, int supp, int hi, int target) {
     int best = -1;
     for (int t = std::max(supp, 8); t <= std::min(hi, n); ++t)
-        if (n % t == 0 && (best < 0 || std::abs(t - 32) < std::abs(best - 32))) best = t;
+        if (n % t == 0 && (best < 0 || std::abs(t - target) < std::abs(best - target))) best = t;
     if (best < 0 && n <= 255) best = n;
     return best;
 }
 
 void exclusive_scan(Workspace& ws, const std::uint32_t* in, std::uint32_t* out, long long n,
                     hipStream_t st) {
-    // level sizes
     std::vector<long long> sizes{n};
     while (sizes.back() > kScanItems) sizes.push_back((sizes.back() + kScanItems - 1) / kScanItems);
     long long total = 0;
@@ -991,12 +1291,12 @@ void exclusive_scan(Workspace& ws, const std::uint32_t* in, std::uint32_t* out, 
         ins.push_back(sums[i - 1]);
         outs.push_back(sums_scanned[i - 1]);
     }
-    std::uint32_t* dummy = tmp + total;  // one slot for the top level's total
+    std::uint32_t* dummy = tmp + total;  // the top level's total
     for (std::size_t lv = 0; lv < sizes.size(); ++lv) {
         const long long m = sizes[lv];
         const long long nblk = (m + kScanItems - 1) / kScanItems;
-        std::uint32_t* s = lv + 1 < sizes.size() ? sums[lv] : dummy;
-        k_scan_block<<<nblk, 1024, 0, st>>>(ins[lv], outs[lv], s, m);
+        std::uint32_t* sm = lv + 1 < sizes.size() ? sums[lv] : dummy;
+        k_scan_block<<<nblk, 1024, 0, st>>>(ins[lv], outs[lv], sm, m);
         HIPCHK(hipGetLastError());
     }
     for (std::size_t lv = sizes.size() - 1; lv-- > 0;) {
@@ -1004,10 +1304,6 @@ void exclusive_scan(Workspace& ws, const std::uint32_t* in, std::uint32_t* out, 
         k_scan_add<<<(m + 255) / 256, 256, 0, st>>>(outs[lv], outs[lv + 1], m);
         HIPCHK(hipGetLastError());
     }
-}
-
-int grid_blocks(long long n) {
-    return int(std::min<long long>((n + 255) / 256, 256 * 64));
 }
 
 template <class T>
@@ -1020,75 +1316,98 @@ struct Tables {
     double* corr = nullptr;
     double* cfu = nullptr;
     double* cfv = nullptr;
+    std::uint32_t* order = nullptr;
 };
-
-template <class T>
-void upload(Workspace& ws, int slot, const std::vector<T>& h, hipStream_t st, T** out) {
-    *out = static_cast<T*>(ws.get(slot, sizeof(T) * std::max<std::size_t>(h.size(), 1)));
-    HIPCHK(hipMemcpyAsync(*out, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
-}
 
 // Host-side values kept alive until the call's stream is synchronised.
 struct HostKeep {
     std::vector<std::vector<double>> d;
     std::vector<std::vector<float>> f;
+    std::vector<std::vector<std::uint32_t>> u;
 };
 
+template <class P>
+P* upload_vec(Workspace& ws, int slot, std::vector<P> h, hipStream_t st, std::vector<std::vector<P>>& keep) {
+    keep.push_back(std::move(h));
+    auto& v = keep.back();
+    P* d = static_cast<P*>(ws.get(slot, sizeof(P) * std::max<std::size_t>(v.size(), 1)));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(d, v.data(), sizeof(P) * v.size(), hipMemcpyHostToDevice, st));
+    return d;
+}
+
 template <class T>
-std::vector<T> cast_vec(const std::vector<double>& v) {
-    return std::vector<T>(v.begin(), v.end());
+T* upload_real(Workspace& ws, int slot, const std::vector<double>& h, hipStream_t st, HostKeep& keep) {
+    if constexpr (sizeof(T) == 8)
+        return reinterpret_cast<T*>(upload_vec<double>(ws, slot, h, st, keep.d));
+    else
+        return reinterpret_cast<T*>(upload_vec<float>(ws, slot, std::vector<float>(h.begin(), h.end()), st, keep.f));
+}
+
+// The signature of everything the tables depend on.
+template <class T>
+std::vector<double> table_key(const Geometry& G, const Geo& g) {
+    return {double(sizeof(T)), double(G.nu), double(G.nv), double(G.nx), double(G.ny),
+            double(G.kernel.index), G.pixsize_x, G.pixsize_y, G.lshift, G.mshift, G.r_l, G.r_m,
+            G.nshift, G.dw, double(G.divide_by_n), double(g.wgrid), double(g.lmshift)};
+}
+
+// Tiles ordered by distance of their centre from the grid centre (u = v = 0,
+// where the visibility density peaks), densest first.
+std::vector<std::uint32_t> tile_order(const Geo& g) {
+    std::vector<std::pair<double, std::uint32_t>> d;
+    d.reserve(std::size_t(g.ntu) * g.ntv);
+    for (int tu = 0; tu < g.ntu; ++tu)
+        for (int tv = 0; tv < g.ntv; ++tv) {
+            const double cu = (tu + 0.5) * g.Tu - 0.5 * g.nu, cv = (tv + 0.5) * g.Tv - 0.5 * g.nv;
+            d.emplace_back(cu * cu + cv * cv, std::uint32_t(tu * g.ntv + tv));
+        }
+    std::stable_sort(d.begin(), d.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<std::uint32_t> out(d.size());
+    for (std::size_t i = 0; i < d.size(); ++i) out[i] = d[i].second;
+    return out;
 }
 
 template <class T>
 void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st, HostKeep& keep,
                  Tables<T>& tb) {
+    const std::vector<double> key = table_key<T>(G, g);
+    const bool cached = ws.table_key == key;
+    tb.twu = static_cast<cx<T>*>(ws.ptr[kSlotTwU]);
+    tb.twv = static_cast<cx<T>*>(ws.ptr[kSlotTwV]);
+    tb.coeff = static_cast<T*>(ws.ptr[kSlotCoeff]);
+    tb.cfu = static_cast<double*>(ws.ptr[kSlotCfu]);
+    tb.cfv = static_cast<double*>(ws.ptr[kSlotCfv]);
+    tb.screen = static_cast<double*>(ws.ptr[kSlotScreen]);
+    tb.corr = static_cast<double*>(ws.ptr[kSlotCorr]);
+    const std::vector<int> okey{g.Tu, g.Tv, g.ntu, g.ntv};
+    if (ws.order_key != okey) {
+        tb.order = upload_vec<std::uint32_t>(ws, kSlotOrder, tile_order(g), st, keep.u);
+        ws.order_key = okey;
+    }
+    tb.order = static_cast<std::uint32_t*>(ws.ptr[kSlotOrder]);
+    if (cached) {
+        tb.du = ws.du;
+        tb.dv = ws.dv;
+        return;
+    }
+    ws.table_key.clear();
     std::vector<double> twu, twv;
     if (!fft_describe(int(G.nu), tb.du, twu) || !fft_describe(int(G.nv), tb.dv, twv))
         throw DevError{"grid size is not a product of 2, 3, 5, 7 and 11"};
-    auto upload_cx = [&](int slot, const std::vector<double>& h, cx<T>** out) {
-        if constexpr (sizeof(T) == 8) {
-            keep.d.push_back(h);
-            *out = static_cast<cx<T>*>(ws.get(slot, sizeof(double) * h.size()));
-            HIPCHK(hipMemcpyAsync(*out, keep.d.back().data(), sizeof(double) * h.size(),
-                                  hipMemcpyHostToDevice, st));
-        } else {
-            keep.f.push_back(cast_vec<float>(h));
-            *out = static_cast<cx<T>*>(ws.get(slot, sizeof(float) * h.size()));
-            HIPCHK(hipMemcpyAsync(*out, keep.f.back().data(), sizeof(float) * h.size(),
-                                  hipMemcpyHostToDevice, st));
-        }
-    };
-    upload_cx(kSlotTwU, twu, &tb.twu);
-    upload_cx(kSlotTwV, twv, &tb.twv);
+    tb.twu = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotTwU, twu, st, keep));
+    tb.twv = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotTwV, twv, st, keep));
     int ne = 0;
-    std::vector<double> eo = G.kernel.even_odd(ne);
-    if constexpr (sizeof(T) == 8) {
-        keep.d.push_back(eo);
-        tb.coeff = static_cast<T*>(ws.get(kSlotCoeff, sizeof(double) * eo.size()));
-        HIPCHK(hipMemcpyAsync(tb.coeff, keep.d.back().data(), sizeof(double) * eo.size(),
-                              hipMemcpyHostToDevice, st));
-    } else {
-        keep.f.push_back(cast_vec<float>(eo));
-        tb.coeff = static_cast<T*>(ws.get(kSlotCoeff, sizeof(float) * eo.size()));
-        HIPCHK(hipMemcpyAsync(tb.coeff, keep.f.back().data(), sizeof(float) * eo.size(),
-                              hipMemcpyHostToDevice, st));
-    }
-    // corfunc tables along u and v (wgridder.rs:824-829)
+    tb.coeff = upload_real<T>(ws, kSlotCoeff, G.kernel.even_odd(ne), st, keep);
+    // correction functions along u and v (wgridder.rs:824-829)
     std::vector<double> cfu(G.nx / 2 + 1), cfv(G.ny / 2 + 1);
     for (std::size_t i = 0; i < cfu.size(); ++i) cfu[i] = G.kernel.corfunc(double(i) * (1.0 / double(G.nu)));
     for (std::size_t i = 0; i < cfv.size(); ++i) cfv[i] = G.kernel.corfunc(double(i) * (1.0 / double(G.nv)));
-    keep.d.push_back(cfu);
-    tb.cfu = static_cast<double*>(ws.get(kSlotCfu, sizeof(double) * cfu.size()));
-    HIPCHK(hipMemcpyAsync(tb.cfu, keep.d.back().data(), sizeof(double) * cfu.size(), hipMemcpyHostToDevice, st));
-    keep.d.push_back(cfv);
-    tb.cfv = static_cast<double*>(ws.get(kSlotCfv, sizeof(double) * cfv.size()));
-    HIPCHK(hipMemcpyAsync(tb.cfv, keep.d.back().data(), sizeof(double) * cfv.size(), hipMemcpyHostToDevice, st));
+    tb.cfu = upload_vec<double>(ws, kSlotCfu, cfu, st, keep.d);
+    tb.cfv = upload_vec<double>(ws, kSlotCfv, cfv, st, keep.d);
     if (g.wgrid) {
         std::vector<double> gl(G.kernel.corr_x);
         gl.insert(gl.end(), G.kernel.corr_w.begin(), G.kernel.corr_w.end());
-        keep.d.push_back(gl);
-        double* dgl = static_cast<double*>(ws.get(kSlotGL, sizeof(double) * gl.size()));
-        HIPCHK(hipMemcpyAsync(dgl, keep.d.back().data(), sizeof(double) * gl.size(), hipMemcpyHostToDevice, st));
+        double* dgl = upload_vec<double>(ws, kSlotGL, gl, st, keep.d);
         const long long tn = (long long)g.tnx * g.tny;
         tb.screen = static_cast<double*>(ws.get(kSlotScreen, sizeof(double) * tn));
         tb.corr = static_cast<double*>(ws.get(kSlotCorr, sizeof(double) * tn));
@@ -1096,40 +1415,49 @@ void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st,
         k_tables<<<(tn + 255) / 256, 256, 0, st>>>(g, tb.screen, tb.corr, tb.cfu, tb.cfv, dgl, dgl + ngl, ngl);
         HIPCHK(hipGetLastError());
     }
+    ws.du = tb.du;
+    ws.dv = tb.dv;
+    ws.table_key = key;
 }
-
-// The part of both directions before the transforms: foc upload, scan,
-// geometry, keys, sort, records. Returns false (with info zeroed) when no
-// visibility is active.
-template <class T>
-struct Prepared {
-    Geo g;
-    Geometry G;
-    Rec<T>* recs = nullptr;
-    std::uint32_t* off = nullptr;
-    long long nvis = 0;
-};
 
 template <class T, bool USE_VIS>
 unsigned long long run_scan(Workspace& ws, const DevIn& din, const Geo& g, hipStream_t st,
                             double& wlo, double& whi) {
     if (g.n == 0) return 0;
+    const int nblk = int(std::min<long long>((g.n + 255) / 256, kScanBlocks));
     unsigned long long* cnt = static_cast<unsigned long long*>(ws.get(kSlotCounters, 3 * sizeof(unsigned long long)));
-    unsigned long long init[3] = {0ull, 0x7ff0000000000000ull, 0ull};
-    HIPCHK(hipMemcpyAsync(cnt, init, sizeof init, hipMemcpyHostToDevice, st));
-    k_scan<T, USE_VIS><<<grid_blocks(g.n), 256, 0, st>>>(din, g, cnt);
+    char* part = static_cast<char*>(ws.get(kSlotScanPart, 24 * std::size_t(kScanBlocks)));
+    unsigned long long* bcnt = reinterpret_cast<unsigned long long*>(part);
+    double* blo = reinterpret_cast<double*>(part + 8 * kScanBlocks);
+    double* bhi = reinterpret_cast<double*>(part + 16 * kScanBlocks);
+    k_scan<T, USE_VIS><<<nblk, 256, 0, st>>>(din, g, bcnt, blo, bhi);
     HIPCHK(hipGetLastError());
-    unsigned long long res[3];
-    HIPCHK(hipMemcpyAsync(res, cnt, sizeof res, hipMemcpyDeviceToHost, st));
+    k_scan_final<<<1, 256, 0, st>>>(bcnt, blo, bhi, nblk, cnt);
+    HIPCHK(hipGetLastError());
+    unsigned long long* res = static_cast<unsigned long long*>(ws.host(3 * sizeof(unsigned long long)));
+    HIPCHK(hipMemcpyAsync(res, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     std::memcpy(&wlo, &res[1], sizeof(double));
     std::memcpy(&whi, &res[2], sizeof(double));
     return res[0];
 }
 
-// Geometry and tiling into the kernels' Geo.
+// LDS bytes the staged degridding needs for tiles of tu x tv cells.
 template <class T>
-bool fill_geo(const Geometry& G, Geo& g, int nout, std::string& err) {
+std::size_t degrid_lds(int tu, int tv, int supp, int P) {
+    return std::size_t(P) * (tu + supp - 1) * (tv + supp - 1) * sizeof(cx<T>);
+}
+
+constexpr std::size_t kGridLds = 80 * 1024;     // gridding accumulators per workgroup
+constexpr std::uint32_t kGridChunk = 1024;       // visibilities per gridding item
+constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
+constexpr std::size_t kDegridLds = 64 * 1024;   // staged planes per workgroup
+
+// Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
+// tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
+// every plane fits kDegridLds.
+template <class T>
+bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err) {
     g.nx = int(G.nx);
     g.ny = int(G.ny);
     g.nu = int(G.nu);
@@ -1158,10 +1486,17 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, std::string& err) {
     g.divide_by_n = G.divide_by_n ? 1 : 0;
     g.tnx = G.lmshift ? g.nx : g.nx / 2 + 1;
     g.tny = G.lmshift ? g.ny : g.ny / 2 + 1;
-    // tile sides: LDS of one plane of nout outputs within 40 KiB
-    const int hi = int(std::sqrt(40960.0 / (double(nout) * 2.0 * sizeof(T))));
-    g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64));
-    g.Tv = choose_tile(g.nv, g.supp, std::min(hi, 64));
+    const int P = g.wgrid ? g.nplanes : 1;
+    int hi = int(std::sqrt(40960.0 / (double(nout) * 2.0 * sizeof(T))));
+    int target = 32;
+    if (degrid) {
+        // largest side whose staged planes fit, within 48
+        const double side = std::sqrt(double(kDegridLds) / (double(P) * sizeof(cx<T>))) - (g.supp - 1);
+        hi = std::min(48, std::max(int(side), 16));
+        target = hi;
+    }
+    g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64), target);
+    g.Tv = choose_tile(g.nv, g.supp, std::min(hi, 64), target);
     if (g.Tu < 0 || g.Tv < 0) {
         err = "no tile size divides the grid";
         return false;
@@ -1191,16 +1526,17 @@ void sort_records(Workspace& ws, const DevIn& din, const Geo& g, long long nvis,
     Rec<T>* recs = static_cast<Rec<T>*>(ws.get(kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
     HIPCHK(hipMemsetAsync(counts, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
     HIPCHK(hipMemsetAsync(cursor, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
-    k_keys<T, USE_VIS><<<grid_blocks(g.n), 256, 0, st>>>(din, g, keys, counts);
+    const int rblocks = (g.nrow + 255) / 256;
+    k_keys<T, USE_VIS><<<rblocks, 256, 0, st>>>(din, g, keys, counts);
     HIPCHK(hipGetLastError());
     exclusive_scan(ws, counts, off, nkeys + 1, st);
-    k_scatter<T, MODE><<<grid_blocks(g.n), 256, 0, st>>>(din, g, keys, off, cursor, recs);
+    k_scatter<T, MODE><<<rblocks, 256, 0, st>>>(din, g, keys, off, cursor, recs);
     HIPCHK(hipGetLastError());
     *recs_out = recs;
     *off_out = off;
 }
 
-Geo base_geo(const DevIn& din, std::size_t nrow, std::size_t nchan, const Options& o) {
+Geo base_geo(std::size_t nrow, std::size_t nchan, const Options& o) {
     Geo g{};
     g.n = (long long)(nrow * nchan);
     g.nrow = int(nrow);
@@ -1212,7 +1548,6 @@ Geo base_geo(const DevIn& din, std::size_t nrow, std::size_t nchan, const Option
     g.fu = o.flip_u ? -1.0 : 1.0;
     g.fv = o.flip_v ? -1.0 : 1.0;
     g.fw = !o.flip_w ? -1.0 : 1.0;  // the core's w flip (wgridder.rs:2461)
-    (void)din;
     return g;
 }
 
@@ -1228,19 +1563,53 @@ bool common_checks(const Options& o, const DevInputs& in, const double* freq_hos
         err = "too many visibilities for one call (at most 2^31 - 1)";
         return false;
     }
-    if (!check_freq(freq_host, in.nchan, err)) return false;
-    return true;
+    return check_freq(freq_host, in.nchan, err);
 }
 
 double* upload_foc(Workspace& ws, const double* freq_host, std::size_t nchan, hipStream_t st,
                    HostKeep& keep) {
     std::vector<double> foc(nchan);
     for (std::size_t c = 0; c < nchan; ++c) foc[c] = freq_host[c] / kCLight;
-    keep.d.push_back(std::move(foc));
     double* d = static_cast<double*>(ws.get(kSlotFoc, sizeof(double) * nchan));
+    if (ws.foc_key == foc) return d;
+    ws.foc_key = foc;
+    keep.d.push_back(std::move(foc));
     HIPCHK(hipMemcpyAsync(d, keep.d.back().data(), sizeof(double) * nchan, hipMemcpyHostToDevice, st));
     return d;
 }
+
+template <int V>
+using ic = std::integral_constant<int, V>;
+
+// Calls f(NT, MAXTOT) with the smallest in-LDS transform size holding `need`
+// values and its thread count.
+template <class T, class F>
+void fft_dispatch(int need, F&& f) {
+    if (need <= 1024)
+        f(ic<256>{}, ic<1024>{});
+    else if (need <= 2048)
+        f(ic<256>{}, ic<2048>{});
+    else if (need <= 4096)
+        f(ic<256>{}, ic<4096>{});
+    else if (need <= 8192)
+        f(ic<512>{}, ic<8192>{});
+    else if constexpr (sizeof(T) == 4) {
+        if (need <= 16384)
+            f(ic<512>{}, ic<16384>{});
+        else
+            throw DevError{"transform too long for LDS"};
+    } else {
+        throw DevError{"transform too long for LDS"};
+    }
+}
+
+template <class K>
+void set_lds(K kernel, std::size_t bytes) {
+    HIPCHK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+// Column block of the u transforms: as many columns as fit 8192 values.
+int col_block(int nu) { return std::max(1, std::min(8, 8192 / nu)); }
 
 }  // namespace
 
@@ -1265,7 +1634,7 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         HostKeep keep;
         StageTimer timer(st);
         DevIn din{in.uvw, nullptr, in.vis, in.wgt, in.wgt2, in.mask, in.vis_kind, in.wgt_kind};
-        Geo g = base_geo(din, in.nrow, in.nchan, o);
+        Geo g = base_geo(in.nrow, in.nchan, o);
         din.foc = upload_foc(ws, freq_host, in.nchan, st, keep);
         const bool use_vis = psf == nullptr;
         double wlo = 0, whi = 0;
@@ -1273,17 +1642,17 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
                                                 : run_scan<T, false>(ws, din, g, st, wlo, whi);
         timer.mark("scan");
         const std::size_t npix = o.nx * o.ny;
-        if (dirty) HIPCHK(hipMemsetAsync(dirty, 0, sizeof(T) * npix, st));
-        if (psf) HIPCHK(hipMemsetAsync(psf, 0, sizeof(T) * npix, st));
         std::fill(info, info + kNInfo, 0.0);
         if (nvis == 0) {
+            if (dirty) HIPCHK(hipMemsetAsync(dirty, 0, sizeof(T) * npix, st));
+            if (psf) HIPCHK(hipMemsetAsync(psf, 0, sizeof(T) * npix, st));
             HIPCHK(hipStreamSynchronize(st));
             return 0;
         }
         Geometry G;
         if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
         const int nout = (dirty ? 1 : 0) + (psf ? 1 : 0);
-        if (!fill_geo<T>(G, g, nout, err)) return 1;
+        if (!fill_geo<T>(G, g, nout, false, err)) return 1;
         Rec<T>* recs;
         std::uint32_t* off;
         if (use_vis)
@@ -1297,29 +1666,54 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         const int P = g.wgrid ? g.nplanes : 1;
         const long long plane = (long long)g.nu * g.nv;
         cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
-        const std::size_t per_plane = std::size_t(nout) * g.Tu * g.Tv * 2 * sizeof(T);
-        if (per_plane > 48 * 1024) throw DevError{"tile too large for LDS"};
-        const int G_planes = std::max(1, std::min(P, int((48 * 1024) / per_plane)));
-        const dim3 ggrid(g.ntu * g.ntv, (P + G_planes - 1) / G_planes);
+        // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
+        const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
+        const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(T);
+        if (per_plane > kGridLds) throw DevError{"tile too large for LDS"};
+        const int G_planes = std::max(1, std::min(P, int(kGridLds / per_plane)));
+        const int ngroups = (P + G_planes - 1) / G_planes;
+        const int ntg = g.ntu * g.ntv * ngroups;
         const std::size_t lds = per_plane * std::min(P, G_planes);
+        std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
+        std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
+        k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, kGridChunk, items);
+        HIPCHK(hipGetLastError());
+        exclusive_scan(ws, items, item_off, ntg + 1, st);
         if (nout == 2)
-            launch_grid<T, 2>(g.supp, ggrid, lds, st, g, recs, off, tb.coeff, din, G_planes, grid);
+            k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
         else
-            launch_grid<T, 1>(g.supp, ggrid, lds, st, g, recs, off, tb.coeff, din, G_planes, grid);
+            k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
+        HIPCHK(hipGetLastError());
+        // every record reaches at most four tiles; each (tile, group) has one item or more
+        const long long ub = ntg + (4LL * ngroups * (long long)nvis + kGridChunk - 1) / kGridChunk;
+        if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
+        if (nout == 2)
+            launch_grid<T, 2>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes, ngroups,
+                              TvP, kGridChunk, item_off, tb.order, grid);
+        else
+            launch_grid<T, 1>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes, ngroups,
+                              TvP, kGridChunk, item_off, tb.order, grid);
         timer.mark("grid");
         cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
         T* proj = static_cast<T*>(ws.get(kSlotProj, sizeof(T) * (long long)P * npix));
-        const int C = std::max(1, std::min(4, fft_maxtot<T>() / g.nu));
+        const int C = col_block(g.nu);
         T* outs[2] = {dirty ? dirty : psf, psf};
         for (int oi = 0; oi < nout; ++oi) {
             const cx<T>* gsrc = grid + (long long)oi * P * plane;
-            const std::size_t ldsv = sizeof(cx<T>) * g.nv, ldsu = sizeof(cx<T>) * g.nu * C;
-            HIPCHK(hipFuncSetAttribute((const void*)k_rows_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsv)));
-            k_rows_inv<T><<<(long long)P * g.nu, kFftThreads, ldsv, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+            fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+                set_lds(k_rows_inv<T, NT, MT>, lds_v);
+                k_rows_inv<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+            });
             HIPCHK(hipGetLastError());
             timer.mark("fft_v");
-            HIPCHK(hipFuncSetAttribute((const void*)k_cols_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsu)));
-            k_cols_inv<T><<<dim3((g.ny + C - 1) / C, P), kFftThreads, ldsu, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+            fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+                set_lds(k_cols_inv<T, NT, MT>, lds_u);
+                k_cols_inv<T, NT, MT><<<dim3((g.ny + C - 1) / C, P), NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+            });
             HIPCHK(hipGetLastError());
             timer.mark("fft_u");
             k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
@@ -1350,7 +1744,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         HostKeep keep;
         StageTimer timer(st);
         DevIn din{in.uvw, nullptr, nullptr, in.wgt, nullptr, in.mask, kVisUnit, in.wgt_kind};
-        Geo g = base_geo(din, in.nrow, in.nchan, o);
+        Geo g = base_geo(in.nrow, in.nchan, o);
         din.foc = upload_foc(ws, freq_host, in.nchan, st, keep);
         double wlo = 0, whi = 0;
         const unsigned long long nvis = run_scan<T, false>(ws, din, g, st, wlo, whi);
@@ -1363,7 +1757,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         }
         Geometry G;
         if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
-        if (!fill_geo<T>(G, g, 1, err)) return 1;
+        if (!fill_geo<T>(G, g, 1, true, err)) return 1;
         Rec<T>* recs;
         std::uint32_t* off;
         sort_records<T, 2, false>(ws, din, g, (long long)nvis, st, &recs, &off);
@@ -1379,17 +1773,38 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         HIPCHK(hipGetLastError());
         cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
         cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
-        const int C = std::max(1, std::min(4, fft_maxtot<T>() / g.nu));
-        const std::size_t ldsv = sizeof(cx<T>) * g.nv, ldsu = sizeof(cx<T>) * g.nu * C;
-        HIPCHK(hipFuncSetAttribute((const void*)k_cols_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsu)));
-        k_cols_fwd<T><<<dim3((g.ny + C - 1) / C, P), kFftThreads, ldsu, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+        const int C = col_block(g.nu);
+        fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+            set_lds(k_cols_fwd<T, NT, MT>, lds_u);
+            k_cols_fwd<T, NT, MT><<<dim3((g.ny + C - 1) / C, P), NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+        });
         HIPCHK(hipGetLastError());
         timer.mark("fft_u");
-        HIPCHK(hipFuncSetAttribute((const void*)k_rows_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsv)));
-        k_rows_fwd<T><<<(long long)P * g.nu, kFftThreads, ldsv, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
+        fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+            set_lds(k_rows_fwd<T, NT, MT>, lds_v);
+            k_rows_fwd<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
+        });
         HIPCHK(hipGetLastError());
         timer.mark("fft_v");
-        launch_degrid<T>(g.supp, g.ntu * g.ntv, st, g, recs, off, tb.coeff, grid, reinterpret_cast<cx<T>*>(vis));
+        const int nt = g.ntu * g.ntv;
+        std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
+        std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
+        k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, kDegridChunk, items);
+        HIPCHK(hipGetLastError());
+        exclusive_scan(ws, items, item_off, nt + 1, st);
+        const long long ub = nt + ((long long)nvis + kDegridChunk - 1) / kDegridChunk;
+        const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
+        cx<T>* cvis = reinterpret_cast<cx<T>*>(vis);
+        if (dl <= kDegridLds)
+            launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, kDegridChunk,
+                                   item_off, tb.order, cvis);
+        else
+            launch_degrid<T, false>(g.supp, int(ub), 0, st, g, recs, off, tb.coeff, grid, kDegridChunk,
+                                    item_off, tb.order, cvis);
         timer.mark("degrid");
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
