@@ -386,42 +386,60 @@ __global__ __launch_bounds__(256) void k_scan_final(const unsigned long long* bc
 
 // Bucket of an active visibility: (tile, spill class, first plane). The
 // spill class says whether its footprint reaches the next tile in u (2) and
-// in v (1). One thread per row walks its channels; a run of channels in the
-// same bucket (the usual case on short baselines) costs one atomic.
+// in v (1).
+template <class T, bool USE_VIS>
+__device__ __forceinline__ std::uint32_t key_of(const DevIn& in, const Geo& g, unsigned i,
+                                                unsigned row, unsigned ch) {
+    if (!active<T, USE_VIS>(in, i)) return kNoKey;
+    double c[3];
+    coord_of(in, g, int(row), c);
+    const double sgn = c[2] < 0.0 ? -1.0 : 1.0;
+    const double f = in.foc[ch];
+    const double u = c[0] * sgn * f, v = c[1] * sgn * f, w = c[2] * sgn * f;
+    double uf, vf;
+    long long iu0, iv0;
+    getpix(g, u, v, uf, vf, iu0, iv0);
+    const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
+    const int tu = su / g.Tu, tv = sv / g.Tv;
+    const int ou = su - tu * g.Tu, ov = sv - tv * g.Tv;
+    const int cat = ((ou + g.supp > g.Tu) ? 2 : 0) | ((ov + g.supp > g.Tv) ? 1 : 0);
+    return std::uint32_t((((long long)tu * g.ntv + tv) * 4 + cat) * g.nb + minplane(g, w));
+}
+
+// Runs of equal keys among the consecutive lanes of a wave (channels of one
+// row usually share a bucket): whether this lane starts a run, the run's
+// length (for the first lane) and this lane's position in its run.
+__device__ __forceinline__ void wave_runs(std::uint32_t key, bool& head, int& len, int& pos,
+                                          int& headlane) {
+    const int lane = threadIdx.x & 63;
+    const std::uint32_t prev = __shfl_up(key, 1, 64);
+    head = lane == 0 || prev != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    headlane = 63 - __clzll(heads & upto);
+    pos = lane - headlane;
+    const unsigned long long after = heads & ~upto;
+    len = (after ? __ffsll((long long)after) - 1 : 64) - lane;
+}
+
+// One thread per visibility (coalesced); the first lane of each run adds the
+// run's length to its bucket's count.
 template <class T, bool USE_VIS>
 __global__ __launch_bounds__(256) void k_keys(DevIn in, Geo g, std::uint32_t* keys,
                                               std::uint32_t* counts) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= g.nrow) return;
-    double c[3];
-    coord_of(in, g, row, c);
-    const double sgn = c[2] < 0.0 ? -1.0 : 1.0;
-    std::uint32_t run_key = kNoKey, run = 0;
-    const unsigned base = unsigned(row) * unsigned(g.nchan);
-    for (int ch = 0; ch < g.nchan; ++ch) {
-        const unsigned i = base + unsigned(ch);
-        std::uint32_t key = kNoKey;
-        if (active<T, USE_VIS>(in, i)) {
-            const double f = in.foc[ch];
-            const double u = c[0] * sgn * f, v = c[1] * sgn * f, w = c[2] * sgn * f;
-            double uf, vf;
-            long long iu0, iv0;
-            getpix(g, u, v, uf, vf, iu0, iv0);
-            const int su = shifted(iu0, g.nu), sv = shifted(iv0, g.nv);
-            const int tu = su / g.Tu, tv = sv / g.Tv;
-            const int ou = su - tu * g.Tu, ov = sv - tv * g.Tv;
-            const int cat = ((ou + g.supp > g.Tu) ? 2 : 0) | ((ov + g.supp > g.Tv) ? 1 : 0);
-            key = std::uint32_t((((long long)tu * g.ntv + tv) * 4 + cat) * g.nb + minplane(g, w));
-        }
+    const unsigned n = unsigned(g.n), nchan = unsigned(g.nchan);
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inside = i < n;
+    std::uint32_t key = kNoKey;
+    if (inside) {
+        const unsigned row = i / nchan;
+        key = key_of<T, USE_VIS>(in, g, i, row, i - row * nchan);
         keys[i] = key;
-        if (key != run_key) {
-            if (run_key != kNoKey) atomicAdd(&counts[run_key], run);
-            run_key = key;
-            run = 0;
-        }
-        ++run;
     }
-    if (run_key != kNoKey) atomicAdd(&counts[run_key], run);
+    bool head;
+    int len, pos, headlane;
+    wave_runs(key, head, len, pos, headlane);
+    if (head && key != kNoKey) atomicAdd(&counts[key], std::uint32_t(len));
 }
 
 constexpr int kScanItems = 4096;  // per block of 1024 threads
@@ -515,22 +533,20 @@ template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_scatter(DevIn in, Geo g, const std::uint32_t* keys,
                                                  const std::uint32_t* offsets,
                                                  std::uint32_t* cursor, Rec<T>* recs) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= g.nrow) return;
+    const unsigned n = unsigned(g.n), nchan = unsigned(g.nchan);
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const std::uint32_t key = i < n ? keys[i] : kNoKey;
+    bool head;
+    int len, pos, headlane;
+    wave_runs(key, head, len, pos, headlane);
+    std::uint32_t base = 0;
+    if (head && key != kNoKey) base = offsets[key] + atomicAdd(&cursor[key], std::uint32_t(len));
+    base = __shfl(base, headlane, 64);
+    if (key == kNoKey) return;
+    const unsigned row = i / nchan, ch = i - row * nchan;
     double c[3];
-    coord_of(in, g, row, c);
-    const unsigned base = unsigned(row) * unsigned(g.nchan);
-    int ch = 0;
-    while (ch < g.nchan) {
-        const std::uint32_t key = keys[base + ch];
-        int end = ch + 1;
-        while (end < g.nchan && keys[base + end] == key) ++end;
-        if (key != kNoKey) {
-            std::uint32_t slot = offsets[key] + atomicAdd(&cursor[key], std::uint32_t(end - ch));
-            for (int k = ch; k < end; ++k) recs[slot++] = make_rec<T, MODE>(in, g, base + k, k, c, key);
-        }
-        ch = end;
-    }
+    coord_of(in, g, int(row), c);
+    recs[base + pos] = make_rec<T, MODE>(in, g, i, int(ch), c, key);
 }
 
 // ------------------------------------------------------------ kernel taps
@@ -716,7 +732,11 @@ constexpr int kGridWaves = kGridThreads / 64;
 // always address distinct cells, however many visibilities share a position
 // (the channels and times of a short baseline do). NOUT = 2 grids a dirty
 // image and a PSF together (wgridder.rs:2128-2152). Row stride TvP is padded
-// so that the S x S footprint spreads over the banks.
+// so that the S x S footprint spreads over the banks. The accumulators are
+// double in both precisions: LDS adds of doubles run some twenty times faster
+// than those of floats on CDNA4, and the sums only gain accuracy.
+using GridAcc = double;
+
 template <class T, int S, int NOUT>
 __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
                                                        const std::uint32_t* __restrict__ off,
@@ -733,8 +753,10 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
     __shared__ T coef[2 * NE * S];
     __shared__ std::uint32_t s_start[9], s_pre[10];
     __shared__ int s_dou[9], s_dov[9];
-    extern __shared__ unsigned char smem[];
-    T* acc = reinterpret_cast<T*>(smem);
+    // 16-byte aligned: the dynamic segment follows the static arrays above,
+    // and LDS double atomics need naturally aligned addresses.
+    extern __shared__ __align__(16) unsigned char smem[];
+    GridAcc* acc = reinterpret_cast<GridAcc*>(smem);
     const int ntg = g.ntu * g.ntv * ngroups;
     int tg;
     std::uint32_t chunk, nchunks;
@@ -745,7 +767,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
     const int P = g.wgrid ? g.nplanes : 1;
     const int p0 = grp * G, p1 = min(P, p0 + G), np = p1 - p0;
     const int pstride = g.Tu * TvP;  // one plane of one output, one of re / im
-    for (int e = threadIdx.x; e < NOUT * np * 2 * pstride; e += kGridThreads) acc[e] = T(0);
+    for (int e = threadIdx.x; e < NOUT * np * 2 * pstride; e += kGridThreads) acc[e] = GridAcc(0);
     for (int e = threadIdx.x; e < 2 * NE * S; e += kGridThreads) coef[e] = coeff[e];
     const int mlo = g.wgrid ? max(0, p0 - S + 1) : 0;
     const int mhi = g.wgrid ? min(g.nb - 1, p1 - 1) : 0;
@@ -823,9 +845,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
                     const T f = kuv[k] * kw;
 #pragma unroll
                     for (int o = 0; o < NOUT; ++o) {
-                        T* are = acc + ((o * np + (p - p0)) * 2) * pstride + at[k];
-                        atomicAdd(are, val[o].re * f);
-                        atomicAdd(are + pstride, val[o].im * f);
+                        GridAcc* are = acc + ((o * np + (p - p0)) * 2) * pstride + at[k];
+                        atomicAdd(are, GridAcc(val[o].re * f));
+                        atomicAdd(are + pstride, GridAcc(val[o].im * f));
                     }
                 }
             }
@@ -835,9 +857,9 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
                 if (!inside[k]) continue;
 #pragma unroll
                 for (int o = 0; o < NOUT; ++o) {
-                    T* are = acc + (o * 2) * pstride + at[k];
-                    atomicAdd(are, val[o].re * kuv[k]);
-                    atomicAdd(are + pstride, val[o].im * kuv[k]);
+                    GridAcc* are = acc + (o * 2) * pstride + at[k];
+                    atomicAdd(are, GridAcc(val[o].re * kuv[k]));
+                    atomicAdd(are + pstride, GridAcc(val[o].im * kuv[k]));
                 }
             }
         }
@@ -852,14 +874,14 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
         const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
 #pragma unroll
         for (int o = 0; o < NOUT; ++o) {
-            const T* are = acc + ((o * np + pl) * 2) * pstride + i * TvP + j;
+            const GridAcc* are = acc + ((o * np + pl) * 2) * pstride + i * TvP + j;
             cx<T>* dst = grid + (long long)o * P * plane + (long long)(p0 + pl) * plane +
                          (long long)grow * g.nv + gcol;
             if (nchunks == 1) {
-                *dst = cx<T>{are[0], are[pstride]};
+                *dst = cx<T>{T(are[0]), T(are[pstride])};
             } else {
-                atomicAdd(&dst->re, are[0]);
-                atomicAdd(&dst->im, are[pstride]);
+                atomicAdd(&dst->re, T(are[0]));
+                atomicAdd(&dst->im, T(are[pstride]));
             }
         }
     }
@@ -940,7 +962,7 @@ template <class T, int NT, int MAXTOT>
 __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
                                                           const cx<T>* __restrict__ tw, int ny) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nv = d.n;
     const long long row = blockIdx.x;
@@ -959,7 +981,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
                                                           T* __restrict__ proj, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
                                                           const double* __restrict__ screen, int C) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nu = d.n, ny = g.ny, nx = g.nx;
     const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
@@ -1033,7 +1055,7 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
                                                           const double* __restrict__ screen, int C) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nu = d.n, ny = g.ny, nx = g.nx;
     const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
@@ -1068,7 +1090,7 @@ template <class T, int NT, int MAXTOT>
 __global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
                                                           cx<T>* __restrict__ grid, FftDesc d,
                                                           const cx<T>* __restrict__ tw, int ny) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nv = d.n;
     const long long row = blockIdx.x;
@@ -1115,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict_
                                                 cx<T>* __restrict__ vis) {
     constexpr int NE = ne_of<S>();
     __shared__ T coef[2 * NE * S];
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     int rank;
     std::uint32_t chunk, nchunks;
@@ -1487,7 +1509,7 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
     g.tnx = G.lmshift ? g.nx : g.nx / 2 + 1;
     g.tny = G.lmshift ? g.ny : g.ny / 2 + 1;
     const int P = g.wgrid ? g.nplanes : 1;
-    int hi = int(std::sqrt(40960.0 / (double(nout) * 2.0 * sizeof(T))));
+    int hi = int(std::sqrt(40960.0 / (double(nout) * 2.0 * sizeof(GridAcc))));
     int target = 32;
     if (degrid) {
         // largest side whose staged planes fit, within 48
@@ -1526,11 +1548,11 @@ void sort_records(Workspace& ws, const DevIn& din, const Geo& g, long long nvis,
     Rec<T>* recs = static_cast<Rec<T>*>(ws.get(kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
     HIPCHK(hipMemsetAsync(counts, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
     HIPCHK(hipMemsetAsync(cursor, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
-    const int rblocks = (g.nrow + 255) / 256;
-    k_keys<T, USE_VIS><<<rblocks, 256, 0, st>>>(din, g, keys, counts);
+    const int eblocks = int((g.n + 255) / 256);
+    k_keys<T, USE_VIS><<<eblocks, 256, 0, st>>>(din, g, keys, counts);
     HIPCHK(hipGetLastError());
     exclusive_scan(ws, counts, off, nkeys + 1, st);
-    k_scatter<T, MODE><<<rblocks, 256, 0, st>>>(din, g, keys, off, cursor, recs);
+    k_scatter<T, MODE><<<eblocks, 256, 0, st>>>(din, g, keys, off, cursor, recs);
     HIPCHK(hipGetLastError());
     *recs_out = recs;
     *off_out = off;
@@ -1668,7 +1690,7 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
         // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
         const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
-        const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(T);
+        const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
         if (per_plane > kGridLds) throw DevError{"tile too large for LDS"};
         const int G_planes = std::max(1, std::min(P, int(kGridLds / per_plane)));
         const int ngroups = (P + G_planes - 1) / G_planes;

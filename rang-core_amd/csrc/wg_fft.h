@@ -4,7 +4,7 @@
 // X[k] = sum_j x[j] exp(sgn 2 pi i j k / n), sgn -1 forward, +1 inverse, as
 // rustfft's (the reference's FFT, rustfft 6.4.1).
 //
-// Scheme: Stockham autosort, one pass per radix (8, 4, 2, then 3, 5, 7, 11).
+// Scheme: Stockham autosort, one pass per radix (3, 5, 7, 11, then 8, 4, 2).
 // A workgroup holds `ncol` transforms of length n back to back in LDS; each
 // pass reads every value into registers, synchronises and writes the pass's
 // output in place, so LDS holds one copy of the data. Twiddles are a per-length
@@ -53,7 +53,21 @@ struct FftDesc {
     int radix[kFftMaxPass] = {};
     int ns[kFftMaxPass] = {};
     int twoff[kFftMaxPass] = {};
+    // Multipliers of udiv for d = n / radix and d = ns.
+    unsigned mnb[kFftMaxPass] = {};
+    unsigned mns[kFftMaxPass] = {};
 };
+
+// x / d as the high half of x * ceil(2^32 / d), exact for x, d < 2^16 (every
+// index here is below the transform length). d = 1 has no 32-bit multiplier
+// (it would be 2^32) and is taken apart.
+inline unsigned fft_magic(unsigned d) {
+    return d <= 1 ? 0u : unsigned(((1ull << 32) + d - 1) / d);
+}
+
+__host__ __device__ __forceinline__ unsigned udiv(unsigned x, unsigned d, unsigned m) {
+    return d == 1 ? x : unsigned((unsigned long long)x * m >> 32);
+}
 
 // Host: the passes of length n and its twiddle table (double precision).
 inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im) {
@@ -66,21 +80,25 @@ inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im) {
         m /= r;
         return true;
     };
+    // Odd radices first: the first pass writes at a stride of its radix, and
+    // an odd stride spreads over the LDS banks where 8 would not.
+    for (int r : {3, 5, 7, 11})
+        while (m % r == 0)
+            if (!push(r)) return false;
     while (m % 8 == 0)
         if (!push(8)) return false;
     while (m % 4 == 0)
         if (!push(4)) return false;
     while (m % 2 == 0)
         if (!push(2)) return false;
-    for (int r : {3, 5, 7, 11})
-        while (m % r == 0)
-            if (!push(r)) return false;
-    if (m != 1) return false;
+    if (m != 1 || n >= 65536) return false;
     int ns = 1, total = 0;
     for (int s = 0; s < d.npass; ++s) {
         d.ns[s] = ns;
         d.twoff[s] = total;
         total += ns * (d.radix[s] - 1);
+        d.mnb[s] = fft_magic(unsigned(n / d.radix[s]));
+        d.mns[s] = fft_magic(unsigned(ns));
         ns *= d.radix[s];
     }
     tw_re_im.assign(std::size_t(2) * (total > 0 ? total : 1), 0.0);
@@ -188,8 +206,8 @@ __device__ __forceinline__ void dft_small(cx<T>* v, int sgn) {
 // One Stockham pass of radix R over `ncol` transforms of length n in LDS.
 // BMAX bounds the butterflies per thread (ceil(ncol n / (R NT))).
 template <class T, int NT, int R, int BMAX>
-__device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, const cx<T>* tw,
-                                         int sgn) {
+__device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, unsigned mnb,
+                                         unsigned mns, const cx<T>* tw, int sgn) {
     const int nb = n / R;
     const int total = nb * ncol;
     cx<T> v[BMAX][R];
@@ -197,12 +215,12 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, co
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
         if (j < total) {
-            const int c = j / nb, jj = j - c * nb;
+            const int c = int(udiv(unsigned(j), unsigned(nb), mnb)), jj = j - c * nb;
             const cx<T>* col = buf + c * n;
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = col[jj + t * nb];
             if (ns > 1) {
-                const int k = jj % ns;
+                const int k = jj - int(udiv(unsigned(jj), unsigned(ns), mns)) * ns;
                 const cx<T>* w = tw + k * (R - 1);
 #pragma unroll
                 for (int t = 1; t < R; ++t) {
@@ -219,10 +237,10 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, co
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
         if (j < total) {
-            const int c = j / nb, jj = j - c * nb;
-            const int k = jj % ns;
+            const int c = int(udiv(unsigned(j), unsigned(nb), mnb)), jj = j - c * nb;
+            const int q = int(udiv(unsigned(jj), unsigned(ns), mns)), k = jj - q * ns;
             cx<T>* col = buf + c * n;
-            const int base = (jj / ns) * ns * R + k;
+            const int base = q * ns * R + k;
 #pragma unroll
             for (int t = 0; t < R; ++t) col[base + t * ns] = v[b][t];
         }
@@ -238,14 +256,15 @@ template <class T, int NT, int MAXTOT>
 __device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
     for (int s = 0; s < d.npass; ++s) {
         const cx<T>* tws = tw + d.twoff[s];
+        const unsigned a = d.mnb[s], b = d.mns[s];
         switch (d.radix[s]) {
-        case 2: fft_pass<T, NT, 2, cdiv(MAXTOT, 2 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        case 3: fft_pass<T, NT, 3, cdiv(MAXTOT, 3 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        case 4: fft_pass<T, NT, 4, cdiv(MAXTOT, 4 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
-        default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT)>(buf, d.n, ncol, d.ns[s], tws, sgn); break;
+        case 2: fft_pass<T, NT, 2, cdiv(MAXTOT, 2 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 3: fft_pass<T, NT, 3, cdiv(MAXTOT, 3 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 4: fft_pass<T, NT, 4, cdiv(MAXTOT, 4 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         }
     }
 }
