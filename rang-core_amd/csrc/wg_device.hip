@@ -617,9 +617,69 @@ __device__ __forceinline__ T w_factor(const Geo& g, const T* c, int nth, int p, 
     return eval_tap<T, S>(c, nth, locz);
 }
 
+// Wave-uniform copies of values every lane holds alike (scalar registers),
+// and a value read from one lane.
+__device__ __forceinline__ std::uint32_t uni(std::uint32_t x) {
+    return std::uint32_t(__builtin_amdgcn_readfirstlane(int(x)));
+}
+__device__ __forceinline__ float uni(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uni(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(b)));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(b >> 32)));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_value(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_value(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b), l));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), l));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One tap's polynomial held in registers: a lane that evaluates the same tap
+// for every visibility loads its even and odd coefficients once, mirrored as
+// eval_tap does, and evaluates with the same operations.
+template <class T, int S>
+struct TapReg {
+    static constexpr int NE = ne_of<S>();
+    T e[NE], o[NE];
+    bool neg;
+    __device__ __forceinline__ void load(const T* c, int nth) {
+        constexpr int H = (S + 1) / 2;
+        neg = nth >= H;
+        if (neg) nth = S - 1 - nth;
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            e[k] = c[k * S + nth];
+            o[k] = c[(NE + k) * S + nth];
+        }
+    }
+    __device__ __forceinline__ T eval(T x) const {
+        if (neg) x = -x;
+        const T x2 = x * x;
+        T a = e[0], b = o[0];
+#pragma unroll
+        for (int k = 1; k < NE; ++k) {
+            a = fma(a, x2, e[k]);
+            b = fma(b, x2, o[k]);
+        }
+        return fma(b, x, a);
+    }
+};
+
 // ------------------------------------------------------------ gridding
 
-constexpr int kGridThreads = 256;
+// Threads per gridding workgroup: one workgroup per CU, 16 waves in single
+// precision (4 per SIMD) and 8 in double, whose kernels need more registers.
+template <class T>
+constexpr int grid_threads() {
+    return sizeof(T) == 4 ? 1024 : 512;
+}
 
 // Work items of the gridding: per (tile rank, plane group) the visibilities
 // of the nine source ranges that reach the tile's cells on the group's
@@ -722,23 +782,44 @@ __device__ __forceinline__ bool find_item(const std::uint32_t* item_off, int nun
     return true;
 }
 
-constexpr int kGridWaves = kGridThreads / 64;
+constexpr int kGridBatch = 512;  // records staged in LDS at a time
 
 // Gridding of one item: a chunk of the visibilities reaching one tile on
 // one group of planes (see k_grid_items), accumulated in LDS and added to the
 // planes (written when the tile has one item, added with global atomics into
-// zeroed cells otherwise). One wave takes one visibility at a time (RPW of
-// them for supports up to 5), one lane per tap: the lanes of an LDS atomic
-// always address distinct cells, however many visibilities share a position
-// (the channels and times of a short baseline do). NOUT = 2 grids a dirty
-// image and a PSF together (wgridder.rs:2128-2152). Row stride TvP is padded
-// so that the S x S footprint spreads over the banks. The accumulators are
-// double in both precisions: LDS adds of doubles run some twenty times faster
-// than those of floats on CDNA4, and the sums only gain accuracy.
+// zeroed cells otherwise). The workgroup stages the chunk's records in LDS a
+// batch at a time, one coalesced load per thread, the next batch prefetched
+// into registers while the current one is gridded. One wave takes one
+// visibility at a time (RPW of them for supports up to 5), one lane per tap:
+// the lanes of an LDS atomic always address distinct cells, however many
+// visibilities share a position (the channels and times of a short baseline
+// do). NOUT = 2 grids a dirty image and a PSF together
+// (wgridder.rs:2128-2152). Row stride TvP is padded so that the S x S
+// footprint spreads over the banks. The accumulators are double in both
+// precisions: LDS adds of doubles run some twenty times faster than those of
+// floats on CDNA4, and the sums only gain accuracy.
 using GridAcc = double;
 
+// Record j of an item's concatenated source ranges, with its in-tile offsets
+// made relative to the item's tile (biased by 64: tiles are at most 64 wide,
+// so the offsets lie in [-64, 64)).
+template <class T>
+__device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restrict__ recs,
+                                             const std::uint32_t* s_start,
+                                             const std::uint32_t* s_pre, std::uint32_t j) {
+    int src = 0;
+    while (j >= s_pre[src + 1]) ++src;
+    Rec<T> r = recs[s_start[src] + (j - s_pre[src])];
+    int du, dv, cat;
+    source_of(src, du, dv, cat);
+    const int ou = int(r.pos & 255u) - (du ? g.Tu : 0) + 64;
+    const int ov = int((r.pos >> 8) & 255u) - (dv ? g.Tv : 0) + 64;
+    r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8) | (r.pos & 0xffff0000u);
+    return r;
+}
+
 template <class T, int S, int NOUT>
-__global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
+__global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
                                                        const std::uint32_t* __restrict__ off,
                                                        const T* __restrict__ coeff, DevIn in,
                                                        int G, int ngroups, int TvP, std::uint32_t rc,
@@ -750,9 +831,10 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
     constexpr int RPW = SS <= 16 ? 4 : SS <= 32 ? 2 : 1;
     constexpr int LPG = 64 / RPW;
     constexpr int CPL = (SS + LPG - 1) / LPG;
+    constexpr int NT = grid_threads<T>(), NW = NT / 64;
     __shared__ T coef[2 * NE * S];
     __shared__ std::uint32_t s_start[9], s_pre[10];
-    __shared__ int s_dou[9], s_dov[9];
+    __shared__ Rec<T> s_rec[kGridBatch];
     // 16-byte aligned: the dynamic segment follows the static arrays above,
     // and LDS double atomics need naturally aligned addresses.
     extern __shared__ __align__(16) unsigned char smem[];
@@ -767,20 +849,15 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
     const int P = g.wgrid ? g.nplanes : 1;
     const int p0 = grp * G, p1 = min(P, p0 + G), np = p1 - p0;
     const int pstride = g.Tu * TvP;  // one plane of one output, one of re / im
-    for (int e = threadIdx.x; e < NOUT * np * 2 * pstride; e += kGridThreads) acc[e] = GridAcc(0);
-    for (int e = threadIdx.x; e < 2 * NE * S; e += kGridThreads) coef[e] = coeff[e];
+    for (int e = threadIdx.x; e < NOUT * np * 2 * pstride; e += NT) acc[e] = GridAcc(0);
+    for (int e = threadIdx.x; e < 2 * NE * S; e += NT) coef[e] = coeff[e];
     const int mlo = g.wgrid ? max(0, p0 - S + 1) : 0;
     const int mhi = g.wgrid ? min(g.nb - 1, p1 - 1) : 0;
     if (threadIdx.x < 9) {
-        const int src = threadIdx.x;
-        int du, dv, cat;
-        source_of(src, du, dv, cat);
         std::uint32_t a, b;
-        source_range(g, off, tu, tv, src, mlo, mhi, a, b);
-        s_start[src] = a;
-        s_pre[src + 1] = b - a;
-        s_dou[src] = du ? -g.Tu : 0;
-        s_dov[src] = dv ? -g.Tv : 0;
+        source_range(g, off, tu, tv, int(threadIdx.x), mlo, mhi, a, b);
+        s_start[threadIdx.x] = a;
+        s_pre[threadIdx.x + 1] = b - a;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -799,67 +876,165 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
         cv[k] = q - (q / S) * S;
     }
     const int srclane = group * LPG;  // lane of the group holding w tap `local`
-    int src = 0;
-    const std::uint32_t step = kGridWaves * RPW;
-    for (std::uint32_t jw = jbeg + wave * RPW; jw < jend; jw += step) {
-        const std::uint32_t j = jw + group;
-        const bool valid = j < jend;
-        const std::uint32_t jj = valid ? j : jw;  // a valid record for every lane
-        while (jj >= s_pre[src + 1]) ++src;
-        const Rec<T> rc_ = recs[s_start[src] + (jj - s_pre[src])];
-        const int ou = int(rc_.pos & 255u) + s_dou[src], ov = int((rc_.pos >> 8) & 255u) + s_dov[src];
-        const int mp = int(rc_.pos >> 16);
-        cx<T> val[NOUT];
-        if constexpr (NOUT == 2) {
-            const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
-            const cx<T> d = cmul(vis_get<T>(in, rc_.idx & ~kFlip), cx<T>{rc_.re, rc_.im * fl});
-            val[0] = {d.re, d.im * fl};
-            val[1] = {rc_.re, rc_.im};
-        } else {
-            val[0] = {rc_.re, rc_.im};
-        }
-        T kuv[CPL];
-        int at[CPL];
-        bool inside[CPL];
+    // With one tap per lane its three polynomials live in registers.
+    constexpr bool REG = CPL == 1;
+    TapReg<T, S> tru, trv, trw;
+    if constexpr (REG) {
+        tru.load(coef, cu[0]);
+        trv.load(coef, cv[0]);
+        trw.load(coef, min(local, S - 1));
+    }
+    const bool stager = threadIdx.x < kGridBatch;
+    Rec<T> nxt{};
+    if (stager && jbeg + threadIdx.x < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x);
+    for (std::uint32_t base = jbeg; base < jend; base += kGridBatch) {
+        const int nbat = int(min(jend - base, std::uint32_t(kGridBatch)));
+        __syncthreads();  // the previous batch is consumed
+        if (int(threadIdx.x) < nbat) s_rec[threadIdx.x] = nxt;
+        __syncthreads();
+        const std::uint32_t jn = base + kGridBatch + threadIdx.x;
+        if (stager && jn < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jn);
+        if constexpr (REG && RPW == 1) {
+            if (g.wgrid) {
+                // One visibility per wave: its fields are wave-uniform, the
+                // plane tests scalar, and the w taps are read from lanes
+                // 0..S-1 into scalar registers.
+                for (int r = wave; r < nbat; r += NW) {
+                    const Rec<T>& rc_ = s_rec[r];
+                    const std::uint32_t pos = uni(rc_.pos);
+                    const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
+                    const int mp = int(pos >> 16);
+                    const T x0 = uni(rc_.x0), y0 = uni(rc_.y0);
+                    const double w = uni(rc_.w);
+                    cx<T> val[NOUT];
+                    if constexpr (NOUT == 2) {
+                        const std::uint32_t idx = uni(rc_.idx);
+                        const T re = uni(rc_.re), im = uni(rc_.im);
+                        const T fl = (idx & kFlip) ? T(-1) : T(1);
+                        const cx<T> d = cmul(vis_get<T>(in, idx & ~kFlip), cx<T>{re, im * fl});
+                        val[0] = {d.re, d.im * fl};
+                        val[1] = {re, im};
+                    } else {
+                        val[0] = {uni(rc_.re), uni(rc_.im)};
+                    }
+                    const GridAcc kd = GridAcc(tru.eval(x0) * trv.eval(y0));
+                    const int iu = ou + cu[0], iv = ov + cv[0];
+                    const bool inside = local < SS && unsigned(iu) < unsigned(g.Tu) &&
+                                        unsigned(iv) < unsigned(g.Tv);
+                    const int at = iu * TvP + iv;
+                    GridAcc a[NOUT], b[NOUT];
 #pragma unroll
-        for (int k = 0; k < CPL; ++k) {
-            kuv[k] = eval_tap<T, S>(coef, cu[k], rc_.x0) * eval_tap<T, S>(coef, cv[k], rc_.y0);
-            const int iu = ou + cu[k], iv = ov + cv[k];
-            inside[k] = valid && local + k * LPG < SS && unsigned(iu) < unsigned(g.Tu) &&
-                        unsigned(iv) < unsigned(g.Tv);
-            at[k] = iu * TvP + iv;
-        }
-        if (g.wgrid) {
-            // lane `local` < S evaluates the w tap `local` (plane mp + local)
-            const int nth = min(local, S - 1);
-            const T kwv = w_factor<T, S>(g, coef, nth, mp + nth, rc_.w);
+                    for (int o = 0; o < NOUT; ++o) {
+                        a[o] = GridAcc(val[o].re) * kd;
+                        b[o] = GridAcc(val[o].im) * kd;
+                    }
+                    // lane `local` < S evaluates the w tap `local` (plane mp + local),
+                    // as w_factor does
+                    const int nth = min(local, S - 1);
+                    const double w0 = g.wmin + double(mp + nth) * g.dw;
+                    const T z = T(g.xdw * (w0 - w));
+                    const T kwv = trw.eval((z - T(nth)) * T(2) + T(double(S) - 1.0));
+                    GridAcc kw[S];
 #pragma unroll
-            for (int nth2 = 0; nth2 < S; ++nth2) {
-                const T kw = __shfl(kwv, srclane + nth2, 64);
-                const int p = mp + nth2;
-                if (p < p0 || p >= p1) continue;
+                    for (int t = 0; t < S; ++t) kw[t] = GridAcc(lane_value(kwv, t));
+                    if (inside) {
+#pragma unroll
+                        for (int t = 0; t < S; ++t) {
+                            const int p = mp + t;
+                            if (p < p0 || p >= p1) continue;
+#pragma unroll
+                            for (int o = 0; o < NOUT; ++o) {
+                                GridAcc* are = acc + ((o * np + (p - p0)) * 2) * pstride + at;
+                                atomicAdd(are, a[o] * kw[t]);
+                                atomicAdd(are + pstride, b[o] * kw[t]);
+                            }
+                        }
+                    }
+                }
+                continue;
+            }
+        }
+        for (int rw = wave * RPW; rw < nbat; rw += NW * RPW) {
+            const int r = rw + group;
+            const bool valid = r < nbat;
+            const Rec<T>& rc_ = s_rec[valid ? r : rw];
+            const std::uint32_t pos = rc_.pos;
+            const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
+            const int mp = int(pos >> 16);
+            const T x0 = rc_.x0, y0 = rc_.y0;
+            cx<T> val[NOUT];
+            if constexpr (NOUT == 2) {
+                const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
+                const cx<T> d = cmul(vis_get<T>(in, rc_.idx & ~kFlip), cx<T>{rc_.re, rc_.im * fl});
+                val[0] = {d.re, d.im * fl};
+                val[1] = {rc_.re, rc_.im};
+            } else {
+                val[0] = {rc_.re, rc_.im};
+            }
+            T kuv[CPL];
+            int at[CPL];
+            bool inside[CPL];
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                if constexpr (REG)
+                    kuv[k] = tru.eval(x0) * trv.eval(y0);
+                else
+                    kuv[k] = eval_tap<T, S>(coef, cu[k], x0) * eval_tap<T, S>(coef, cv[k], y0);
+                const int iu = ou + cu[k], iv = ov + cv[k];
+                inside[k] = valid && local + k * LPG < SS && unsigned(iu) < unsigned(g.Tu) &&
+                            unsigned(iv) < unsigned(g.Tv);
+                at[k] = iu * TvP + iv;
+            }
+            // The products with the taps are formed in the accumulators'
+            // precision (as fast as single precision on CDNA4, and no
+            // conversion per plane).
+            GridAcc vre[NOUT], vim[NOUT], kd[CPL];
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) {
+                vre[o] = GridAcc(val[o].re);
+                vim[o] = GridAcc(val[o].im);
+            }
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) kd[k] = GridAcc(kuv[k]);
+            if (g.wgrid) {
+                // lane `local` < S evaluates the w tap `local` (plane mp + local)
+                const int nth = min(local, S - 1);
+                T kwv;
+                if constexpr (REG) {
+                    // w_factor's argument with the lane's polynomial
+                    const double w0 = g.wmin + double(mp + nth) * g.dw;
+                    const T z = T(g.xdw * (w0 - rc_.w));
+                    kwv = trw.eval((z - T(nth)) * T(2) + T(double(S) - 1.0));
+                } else {
+                    kwv = w_factor<T, S>(g, coef, nth, mp + nth, rc_.w);
+                }
+#pragma unroll
+                for (int nth2 = 0; nth2 < S; ++nth2) {
+                    const GridAcc kw = GridAcc(__shfl(kwv, srclane + nth2, 64));
+                    const int p = mp + nth2;
+                    if (p < p0 || p >= p1) continue;
+#pragma unroll
+                    for (int k = 0; k < CPL; ++k) {
+                        if (!inside[k]) continue;
+                        const GridAcc f = kd[k] * kw;
+#pragma unroll
+                        for (int o = 0; o < NOUT; ++o) {
+                            GridAcc* are = acc + ((o * np + (p - p0)) * 2) * pstride + at[k];
+                            atomicAdd(are, vre[o] * f);
+                            atomicAdd(are + pstride, vim[o] * f);
+                        }
+                    }
+                }
+            } else {
 #pragma unroll
                 for (int k = 0; k < CPL; ++k) {
                     if (!inside[k]) continue;
-                    // (ku kw) kv as the reference forms ku * kw first
-                    const T f = kuv[k] * kw;
 #pragma unroll
                     for (int o = 0; o < NOUT; ++o) {
-                        GridAcc* are = acc + ((o * np + (p - p0)) * 2) * pstride + at[k];
-                        atomicAdd(are, GridAcc(val[o].re * f));
-                        atomicAdd(are + pstride, GridAcc(val[o].im * f));
+                        GridAcc* are = acc + (o * 2) * pstride + at[k];
+                        atomicAdd(are, vre[o] * kd[k]);
+                        atomicAdd(are + pstride, vim[o] * kd[k]);
                     }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < CPL; ++k) {
-                if (!inside[k]) continue;
-#pragma unroll
-                for (int o = 0; o < NOUT; ++o) {
-                    GridAcc* are = acc + (o * 2) * pstride + at[k];
-                    atomicAdd(are, GridAcc(val[o].re * kuv[k]));
-                    atomicAdd(are + pstride, GridAcc(val[o].im * kuv[k]));
                 }
             }
         }
@@ -867,7 +1042,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Geo g, const Rec<T>* __re
     __syncthreads();
     const long long plane = (long long)g.nu * g.nv;
     const int cells = g.Tu * g.Tv;
-    for (int e = threadIdx.x; e < np * cells; e += kGridThreads) {
+    for (int e = threadIdx.x; e < np * cells; e += NT) {
         const int pl = e / cells, rem = e - pl * cells;
         const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
         const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
@@ -956,6 +1131,17 @@ __device__ __forceinline__ cx<T> screen_at(double w, double s, bool adjoint) {
     return {cs, sn};
 }
 
+// A linear block index remapped so that the workgroups one XCD holds at a time
+// take consecutive indices: the dispatcher deals consecutive workgroups to the
+// eight XCDs in turn, and each XCD has its own L2, so blocks that read
+// neighbouring columns should share an XCD.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    constexpr int X = 8;
+    const int per = nblocks / X, rem = nblocks - per * X;
+    const int x = bid % X, k = bid / X;
+    return x * per + min(x, rem) + k;
+}
+
 // v2d: inverse transform along v of one grid row, keeping the image columns
 // (wgridder.rs:2331-2338, 1745-1749).
 template <class T, int NT, int MAXTOT>
@@ -984,8 +1170,10 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nu = d.n, ny = g.ny, nx = g.nx;
-    const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
-    const int p = blockIdx.y;
+    const int ncb = (ny + C - 1) / C;
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lin / ncb;
+    const int j0 = (lin - p * ncb) * C, nc = min(C, ny - j0);
     const cx<T>* src = tmp + (long long)p * nu * ny;
     for (int e = threadIdx.x; e < nu * nc; e += NT) {
         const int u = e / nc, c = e - u * nc;
@@ -1058,8 +1246,10 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nu = d.n, ny = g.ny, nx = g.nx;
-    const int j0 = blockIdx.x * C, nc = min(C, ny - j0);
-    const int p = blockIdx.y;
+    const int ncb = (ny + C - 1) / C;
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lin / ncb;
+    const int j0 = (lin - p * ncb) * C, nc = min(C, ny - j0);
     for (int e = threadIdx.x; e < nu * nc; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
     const double w0 = g.wmin + double(p) * g.dw;
@@ -1236,7 +1426,7 @@ void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo&
         if constexpr (SS <= max_support<T>()) {                                                 \
             HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT>,                        \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
-            k_grid<T, SS, NOUT><<<nblocks, kGridThreads, lds, st>>>(g, recs, off, coeff, in, G,  \
+            k_grid<T, SS, NOUT><<<nblocks, grid_threads<T>(), lds, st>>>(g, recs, off, coeff, in, G,  \
                                                                      ngroups, TvP, rc, item_off, \
                                                                      order, out);               \
         } else {                                                                                \
@@ -1284,11 +1474,14 @@ void launch_degrid(int S, int nblocks, std::size_t lds, hipStream_t st, const Ge
 
 // Tile side along a grid axis of n cells: a divisor of n (so that no tile
 // is partial) of at least the support and at most `hi`, nearest `target`.
+// Tile side: the divisor of n in [max(supp, 8), hi] nearest `target`, else
+// the smallest one up to 64 (the kernels keep in-tile offsets below 64).
 int choose_tile(int n, int supp, int hi, int target) {
     int best = -1;
     for (int t = std::max(supp, 8); t <= std::min(hi, n); ++t)
         if (n % t == 0 && (best < 0 || std::abs(t - target) < std::abs(best - target))) best = t;
-    if (best < 0 && n <= 255) best = n;
+    for (int t = std::max(supp, 8); best < 0 && t <= std::min(64, n); ++t)
+        if (n % t == 0) best = t;
     return best;
 }
 
@@ -1442,6 +1635,14 @@ void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st,
     ws.table_key = key;
 }
 
+// One count from the device (the stream is synchronised).
+long long read_u32(Workspace& ws, const std::uint32_t* d, hipStream_t st) {
+    std::uint32_t* h = static_cast<std::uint32_t*>(ws.host(sizeof(std::uint32_t)));
+    HIPCHK(hipMemcpyAsync(h, d, sizeof(std::uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return (long long)*h;
+}
+
 template <class T, bool USE_VIS>
 unsigned long long run_scan(Workspace& ws, const DevIn& din, const Geo& g, hipStream_t st,
                             double& wlo, double& whi) {
@@ -1470,7 +1671,13 @@ std::size_t degrid_lds(int tu, int tv, int supp, int P) {
     return std::size_t(P) * (tu + supp - 1) * (tv + supp - 1) * sizeof(cx<T>);
 }
 
-constexpr std::size_t kGridLds = 80 * 1024;     // gridding accumulators per workgroup
+// Gridding accumulators per workgroup: one workgroup per CU, the 160 KiB of
+// a gfx950 CU's LDS less the staged records and the largest coefficient
+// table.
+template <class T>
+constexpr std::size_t grid_lds() {
+    return 160 * 1024 - kGridBatch * sizeof(Rec<T>) - 2 * 10 * 16 * sizeof(T) - 256;
+}
 constexpr std::uint32_t kGridChunk = 1024;       // visibilities per gridding item
 constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
 constexpr std::size_t kDegridLds = 64 * 1024;   // staged planes per workgroup
@@ -1691,8 +1898,8 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
         const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
         const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
-        if (per_plane > kGridLds) throw DevError{"tile too large for LDS"};
-        const int G_planes = std::max(1, std::min(P, int(kGridLds / per_plane)));
+        if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
+        const int G_planes = std::max(1, std::min(P, int(grid_lds<T>() / per_plane)));
         const int ngroups = (P + G_planes - 1) / G_planes;
         const int ntg = g.ntu * g.ntv * ngroups;
         const std::size_t lds = per_plane * std::min(P, G_planes);
@@ -1706,8 +1913,7 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         else
             k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
         HIPCHK(hipGetLastError());
-        // every record reaches at most four tiles; each (tile, group) has one item or more
-        const long long ub = ntg + (4LL * ngroups * (long long)nvis + kGridChunk - 1) / kGridChunk;
+        const long long ub = read_u32(ws, item_off + ntg, st);
         if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
         if (nout == 2)
             launch_grid<T, 2>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes, ngroups,
@@ -1734,7 +1940,7 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
                 constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
                 const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
                 set_lds(k_cols_inv<T, NT, MT>, lds_u);
-                k_cols_inv<T, NT, MT><<<dim3((g.ny + C - 1) / C, P), NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+                k_cols_inv<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
             });
             HIPCHK(hipGetLastError());
             timer.mark("fft_u");
@@ -1800,7 +2006,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
             constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
             const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
             set_lds(k_cols_fwd<T, NT, MT>, lds_u);
-            k_cols_fwd<T, NT, MT><<<dim3((g.ny + C - 1) / C, P), NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+            k_cols_fwd<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
         });
         HIPCHK(hipGetLastError());
         timer.mark("fft_u");
@@ -1818,7 +2024,8 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, kDegridChunk, items);
         HIPCHK(hipGetLastError());
         exclusive_scan(ws, items, item_off, nt + 1, st);
-        const long long ub = nt + ((long long)nvis + kDegridChunk - 1) / kDegridChunk;
+        const long long ub = read_u32(ws, item_off + nt, st);
+        if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
         const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
         cx<T>* cvis = reinterpret_cast<cx<T>*>(vis);
         if (dl <= kDegridLds)

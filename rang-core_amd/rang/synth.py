@@ -101,3 +101,55 @@ def config1(seed=1234, ntime=100):
     case = imaging_case(ntime, 16, 1.30e9, 1.45e9, 1024, 2.0, seed=seed)
     case["epsilon"] = 1e-5
     return case
+
+
+def imaging_case_device(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, device, *, nant=64,
+                        radius_m=8000.0, lat_deg=-30.7, dec_deg=-45.0, dt_s=8.0, nsrc=20,
+                        sky_radius_deg=0.5, flux=(1e-3, 1.0), noise_jy=1e-3, seed=1234,
+                        rows_per_chunk=32768):
+    """The recipe of imaging_case with the visibilities formed on a GPU (for
+    workloads whose direct sum is too large for the host): the same array,
+    tracks and sky; the sum in float64 on the device, stored complex64; the
+    noise from torch's generator seeded with seed + 1 (so its values differ
+    from the host recipe's). Returns torch tensors on `device` and the sky."""
+    import torch
+
+    enu = antennas(nant, radius_m, seed)
+    uvw = uvw_tracks(baselines_xyz(enu, lat_deg), ntime, dt_s, dec_deg)
+    freq = np.linspace(f_lo, f_hi, nchan)
+    flux_s, l_s, m_s = point_sky(nsrc, sky_radius_deg, flux[0], flux[1], seed + 7)
+    d_uvw = torch.from_numpy(uvw).to(device)
+    f = torch.from_numpy(freq / C_LIGHT).to(device)
+    vis = torch.empty((uvw.shape[0], nchan), dtype=torch.complex64, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed + 1)
+    for r0 in range(0, uvw.shape[0], rows_per_chunk):
+        r1 = min(uvw.shape[0], r0 + rows_per_chunk)
+        u = d_uvw[r0:r1]
+        acc = torch.zeros((r1 - r0, nchan), dtype=torch.complex128, device=device)
+        for s in range(flux_s.size):
+            n1 = np.sqrt(1.0 - l_s[s] ** 2 - m_s[s] ** 2) - 1.0
+            path = u[:, 0] * l_s[s] + u[:, 1] * m_s[s] + u[:, 2] * n1
+            ph = (-2.0 * np.pi) * path[:, None] * f[None, :]
+            acc += flux_s[s] * torch.polar(torch.ones_like(ph), ph)
+        noise = torch.randn((r1 - r0, nchan, 2), dtype=torch.float64, device=device, generator=gen)
+        acc += noise_jy * torch.complex(noise[..., 0], noise[..., 1])
+        vis[r0:r1] = acc.to(torch.complex64)
+    return {
+        "uvw": d_uvw,
+        "freq": torch.from_numpy(freq).to(device),
+        "vis": vis,
+        "wgt": torch.ones((uvw.shape[0], nchan), dtype=torch.float32, device=device),
+        "npix": npix,
+        "pixsize": pix_arcsec * ARCSEC,
+        "sky": (flux_s, l_s, m_s),
+    }
+
+
+def config2_device(device, seed=1234, ntime=900, nchan=256):
+    """The scaled-up workload: the same array and sky recipe, 2016 baselines x
+    900 times of 8 s x 256 channels over 0.9-1.7 GHz (464,486,400 complex64
+    visibilities), 8192^2 image at 1 arcsec, epsilon 1e-5; formed on the GPU."""
+    case = imaging_case_device(ntime, nchan, 0.9e9, 1.7e9, 8192, 1.0, device, seed=seed)
+    case["epsilon"] = 1e-5
+    return case
