@@ -156,6 +156,69 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions *opts, size_t 
                                             const float *dirty, const void *wgt, int32_t wgt_kind,
                                             const uint8_t *mask, float *vis, double *info,
                                             void *stream);
+/* --- Prepared plans: the reference's Rust-side Prepared (core/src/wgridder.rs:
+ * 3165-3499), which has no C entry point there, on device buffers. ---------
+ * A plan and both sorted layouts of one set of visibilities (uvw nrow x 3,
+ * weights nrow x nchan of kind none / f32 / f64, all device pointers; freq a
+ * device pointer too), kept on the GPU until _free. The active set is the
+ * nonzero weights (Prepared::new, :3182-3228). Returns NULL on failure, with
+ * the message for rang_core_wgridder_last_error. Calls on one plan are
+ * serialised. */
+typedef struct RangPrepared RangPrepared;
+RangPrepared *rang_amd_wgridder_prepare_dev_f64(const WgridderOptions *opts, size_t nrow,
+                                                size_t nchan, const double *uvw,
+                                                const double *freq, const void *wgt,
+                                                int32_t wgt_kind, void *stream);
+RangPrepared *rang_amd_wgridder_prepare_dev_f32(const WgridderOptions *opts, size_t nrow,
+                                                size_t nchan, const double *uvw,
+                                                const double *freq, const void *wgt,
+                                                int32_t wgt_kind, void *stream);
+/* Prepared::vis2dirty (:3432-3456): the image of `vis` (kind c64 / c128, nrow x
+ * nchan) with the prepared weights; info as the gridder's. */
+int32_t rang_amd_wgridder_prepared_vis2dirty_f64(RangPrepared *p, const void *vis,
+                                                 int32_t vis_kind, double *dirty, double *info,
+                                                 void *stream);
+int32_t rang_amd_wgridder_prepared_vis2dirty_f32(RangPrepared *p, const void *vis,
+                                                 int32_t vis_kind, float *dirty, double *info,
+                                                 void *stream);
+/* Prepared::dirty2vis (:3458-3486): complex visibilities (interleaved re, im)
+ * of the image, times the prepared weights when `weighted`; zero where the
+ * weight is zero. */
+int32_t rang_amd_wgridder_prepared_dirty2vis_f64(RangPrepared *p, const double *dirty,
+                                                 int32_t weighted, double *vis, void *stream);
+int32_t rang_amd_wgridder_prepared_dirty2vis_f32(RangPrepared *p, const float *dirty,
+                                                 int32_t weighted, float *vis, void *stream);
+/* Prepared::residual (:3230-3255, 3300-3383): the image of vis - A model (model
+ * NULL: of vis) with the prepared weights, and misfit = sum w |vis - A model|^2
+ * (a parallel sum in double, not the reference's serial order). */
+int32_t rang_amd_wgridder_prepared_residual_f64(RangPrepared *p, const void *vis,
+                                                int32_t vis_kind, const double *model,
+                                                double *dirty, double *misfit, void *stream);
+int32_t rang_amd_wgridder_prepared_residual_f32(RangPrepared *p, const void *vis,
+                                                int32_t vis_kind, const float *model,
+                                                float *dirty, double *misfit, void *stream);
+/* The normal operator A^H W A x of the prepared plan (WgridMeasurement::
+ * gram_apply, core/src/clean_certified.rs:233-236): out = vis2dirty(dirty2vis(x)). */
+int32_t rang_amd_wgridder_prepared_gram_f64(RangPrepared *p, const double *x, double *out,
+                                            void *stream);
+int32_t rang_amd_wgridder_prepared_gram_f32(RangPrepared *p, const float *x, float *out,
+                                            void *stream);
+/* clean::conjugate_gradients (core/src/clean.rs:263-306) on A^H W A x = rhs from
+ * zero: at most `iterations` Gram applications, stop at |r| <= tolerance |rhs|
+ * or <p, A p> <= 0; *used applications, *rel = |r| / |rhs|. */
+int32_t rang_amd_wgridder_prepared_cg_f64(RangPrepared *p, const double *rhs, double *x,
+                                          int32_t iterations, double tolerance, int32_t *used,
+                                          double *rel, void *stream);
+int32_t rang_amd_wgridder_prepared_cg_f32(RangPrepared *p, const float *rhs, float *x,
+                                          int32_t iterations, double tolerance, int32_t *used,
+                                          double *rel, void *stream);
+/* Prepared::active (:3257-3262), memory (:3386-3418, total device bytes),
+ * release_grids (:3420-3429), and the plan's release. */
+size_t rang_amd_wgridder_prepared_active(const RangPrepared *p);
+size_t rang_amd_wgridder_prepared_memory(const RangPrepared *p);
+void rang_amd_wgridder_prepared_release_grids(RangPrepared *p);
+void rang_amd_wgridder_prepared_free(RangPrepared *p);
+
 /* Per-stage timing (HIP events on the call's stream) of this thread's later
  * calls: on != 0 enables it; count, names and milliseconds of the last call. */
 void rang_amd_stage_timing(int32_t on);

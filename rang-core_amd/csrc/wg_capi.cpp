@@ -308,9 +308,75 @@ int32_t d2v_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan
     }
 }
 
+
+template <class T>
+RangPrepared* prepare_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
+                          const double* uvw, const double* freq, const void* wgt, int32_t wgt_kind,
+                          void* stream) {
+    try {
+        if (!opts) return fail(1, "null options"), nullptr;
+        if (!uvw || !freq) return fail(1, "null array"), nullptr;
+        std::string err;
+        if (!kinds_ok(nullptr, kVisUnit, wgt, nullptr, wgt_kind, err)) return fail(1, err), nullptr;
+        std::vector<double> fh;
+        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err), nullptr;
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = uvw;
+        in.freq = freq;
+        in.wgt = wgt;
+        in.wgt_kind = wgt_kind;
+        int status = 0;
+        PreparedBase* p = prepared_new<T>(to_options(*opts), in, fh.data(),
+                                          static_cast<hipStream_t>(stream), status, err);
+        if (!p) fail(status ? status : 2, err);
+        return reinterpret_cast<RangPrepared*>(p);
+    } catch (const std::exception& e) {
+        fail(2, std::string("failure in the gridder: ") + e.what());
+        return nullptr;
+    }
+}
+
+template <class T>
+PreparedBase* handle(RangPrepared* h, std::string& err) {
+    PreparedBase* p = reinterpret_cast<PreparedBase*>(h);
+    if (!p) {
+        err = "null prepared plan";
+        return nullptr;
+    }
+    if (p->single() != (sizeof(T) == 4)) {
+        err = "prepared plan of the other precision";
+        return nullptr;
+    }
+    return p;
+}
+
+template <class T, class F>
+int32_t prepared_call(RangPrepared* h, F&& f) {
+    try {
+        std::string err;
+        PreparedBase* p = handle<T>(h, err);
+        if (!p) return fail(1, err);
+        const int32_t status = f(p, err);
+        return status ? fail(status, err) : 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+bool vis_ok(const void* vis, int32_t kind, std::string& err) {
+    if (!vis || !(kind == kVisC64 || kind == kVisC128)) {
+        err = "invalid visibility array or kind";
+        return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
+
 
 int32_t rang_core_wgridder_vis2dirty_f64(const WgridderOptions* opts, size_t nrow, size_t nchan,
                                          const double* uvw, const double* freq, const double* vis,
@@ -449,6 +515,79 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions* opts, size_t 
     return d2v_dev<float>(opts, nrow, nchan, uvw, freq, dirty, wgt, wgt_kind, mask, vis, info,
                           stream);
 }
+
+
+#define RANG_PREPARED_ABI(SUF, T)                                                                   \
+    RangPrepared* rang_amd_wgridder_prepare_dev_##SUF(const WgridderOptions* opts, size_t nrow,    \
+                                                      size_t nchan, const double* uvw,            \
+                                                      const double* freq, const void* wgt,        \
+                                                      int32_t wgt_kind, void* stream) {           \
+        return prepare_dev<T>(opts, nrow, nchan, uvw, freq, wgt, wgt_kind, stream);                \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_vis2dirty_##SUF(RangPrepared* h, const void* vis,           \
+                                                       int32_t vis_kind, T* dirty, double* info,   \
+                                                       void* stream) {                             \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!vis_ok(vis, vis_kind, err)) return 1;                                             \
+            if (!dirty || !info) return err = "null array", 1;                                     \
+            return prepared_vis2dirty<T>(p, vis, vis_kind, dirty, info,                            \
+                                         static_cast<hipStream_t>(stream), err);                   \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_dirty2vis_##SUF(RangPrepared* h, const T* dirty,            \
+                                                       int32_t weighted, T* vis, void* stream) {   \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!dirty || !vis) return err = "null array", 1;                                      \
+            return prepared_dirty2vis<T>(p, dirty, weighted != 0, vis,                             \
+                                         static_cast<hipStream_t>(stream), err);                   \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_residual_##SUF(RangPrepared* h, const void* vis,            \
+                                                      int32_t vis_kind, const T* model, T* dirty,  \
+                                                      double* misfit, void* stream) {              \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!vis_ok(vis, vis_kind, err)) return 1;                                             \
+            if (!dirty || !misfit) return err = "null array", 1;                                   \
+            return prepared_residual<T>(p, vis, vis_kind, model, dirty, misfit,                    \
+                                        static_cast<hipStream_t>(stream), err);                    \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_gram_##SUF(RangPrepared* h, const T* x, T* out,             \
+                                                  void* stream) {                                  \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!x || !out) return err = "null array", 1;                                          \
+            return prepared_gram<T>(p, x, out, static_cast<hipStream_t>(stream), err);             \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_cg_##SUF(RangPrepared* h, const T* rhs, T* x,               \
+                                                int32_t iterations, double tolerance,              \
+                                                int32_t* used, double* rel, void* stream) {        \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!rhs || !x || !used || !rel) return err = "null array", 1;                         \
+            int u = 0;                                                                              \
+            const int st = prepared_cg<T>(p, rhs, x, iterations, tolerance, &u, rel,               \
+                                          static_cast<hipStream_t>(stream), err);                  \
+            *used = u;                                                                              \
+            return st;                                                                              \
+        });                                                                                         \
+    }
+RANG_PREPARED_ABI(f64, double)
+RANG_PREPARED_ABI(f32, float)
+#undef RANG_PREPARED_ABI
+
+size_t rang_amd_wgridder_prepared_active(const RangPrepared* h) {
+    const PreparedBase* p = reinterpret_cast<const PreparedBase*>(h);
+    return p ? p->active() : 0;
+}
+size_t rang_amd_wgridder_prepared_memory(const RangPrepared* h) {
+    const PreparedBase* p = reinterpret_cast<const PreparedBase*>(h);
+    return p ? p->memory() : 0;
+}
+void rang_amd_wgridder_prepared_release_grids(RangPrepared* h) {
+    PreparedBase* p = reinterpret_cast<PreparedBase*>(h);
+    if (p) p->release_grids();
+}
+void rang_amd_wgridder_prepared_free(RangPrepared* h) { delete reinterpret_cast<PreparedBase*>(h); }
 
 void rang_amd_stage_timing(int32_t on) { set_stage_timing(on != 0); }
 int32_t rang_amd_stage_count(void) { return stage_timing_count(); }

@@ -39,6 +39,36 @@ template <class T>
 int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
                      const T* dirty, T* vis, double* info, hipStream_t stream, std::string& err);
 
+// A prepared set of visibilities (core/src/wgridder.rs:3165-3499): plan and
+// sorted layouts kept on the device between calls.
+struct PreparedBase {
+    virtual ~PreparedBase() = default;
+    virtual bool single() const = 0;
+    virtual std::size_t active() const = 0;
+    virtual std::size_t memory() const = 0;
+    virtual void release_grids() = 0;
+};
+
+// Device pointers in `in` (uvw, weights f32/f64 or none); freq on the host.
+// Returns null with status 1 (invalid) or 2 (device failure) and a message.
+template <class T>
+PreparedBase* prepared_new(const Options& o, const DevInputs& in, const double* freq_host,
+                           hipStream_t stream, int& status, std::string& err);
+template <class T>
+int prepared_vis2dirty(PreparedBase* p, const void* vis, int vis_kind, T* dirty, double* info,
+                       hipStream_t stream, std::string& err);
+template <class T>
+int prepared_dirty2vis(PreparedBase* p, const T* dirty, bool weighted, T* vis, hipStream_t stream,
+                       std::string& err);
+template <class T>
+int prepared_residual(PreparedBase* p, const void* vis, int vis_kind, const T* model, T* dirty,
+                      double* misfit, hipStream_t stream, std::string& err);
+template <class T>
+int prepared_gram(PreparedBase* p, const T* x, T* out, hipStream_t stream, std::string& err);
+template <class T>
+int prepared_cg(PreparedBase* p, const T* rhs, T* x, int iterations, double tolerance, int* used,
+                double* rel, hipStream_t stream, std::string& err);
+
 // Per-kernel timing of the last call on this thread (hipEvents on the call's
 // stream; enabled by set_stage_timing). Names and milliseconds.
 void set_stage_timing(bool on);

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -76,6 +77,17 @@ enum Slot : int {
     kSlotScanPart,
     kSlotItems,
     kSlotItemOff,
+    // a second sorted set (dirty2vis tiling) and the order of its tiles
+    kSlotKeys2,
+    kSlotCounts2,
+    kSlotOffsets2,
+    kSlotCursor2,
+    kSlotRecs2,
+    kSlotOrder2,
+    // prepared calls: visibility buffer, reductions, solver vectors
+    kSlotVbuf,
+    kSlotRed,
+    kSlotSolve,
     kSlotCount
 };
 
@@ -93,7 +105,7 @@ struct Workspace {
     std::vector<double> table_key;
     FftDesc du, dv;
     std::vector<double> foc_key;
-    std::vector<int> order_key;
+    std::vector<int> order_key[2];
     void* host(std::size_t bytes) {
         if (pinned_size < bytes) {
             if (pinned) HIPCHK(hipHostFree(pinned));
@@ -818,7 +830,7 @@ __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restr
     return r;
 }
 
-template <class T, int S, int NOUT>
+template <class T, int S, int NOUT, bool FORM>
 __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
                                                        const std::uint32_t* __restrict__ off,
                                                        const T* __restrict__ coeff, DevIn in,
@@ -907,13 +919,13 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                     const T x0 = uni(rc_.x0), y0 = uni(rc_.y0);
                     const double w = uni(rc_.w);
                     cx<T> val[NOUT];
-                    if constexpr (NOUT == 2) {
+                    if constexpr (NOUT == 2 || FORM) {
                         const std::uint32_t idx = uni(rc_.idx);
                         const T re = uni(rc_.re), im = uni(rc_.im);
                         const T fl = (idx & kFlip) ? T(-1) : T(1);
                         const cx<T> d = cmul(vis_get<T>(in, idx & ~kFlip), cx<T>{re, im * fl});
                         val[0] = {d.re, d.im * fl};
-                        val[1] = {re, im};
+                        if constexpr (NOUT == 2) val[1] = {re, im};
                     } else {
                         val[0] = {uni(rc_.re), uni(rc_.im)};
                     }
@@ -963,11 +975,11 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
             const int mp = int(pos >> 16);
             const T x0 = rc_.x0, y0 = rc_.y0;
             cx<T> val[NOUT];
-            if constexpr (NOUT == 2) {
+            if constexpr (NOUT == 2 || FORM) {
                 const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
                 const cx<T> d = cmul(vis_get<T>(in, rc_.idx & ~kFlip), cx<T>{rc_.re, rc_.im * fl});
                 val[0] = {d.re, d.im * fl};
-                val[1] = {rc_.re, rc_.im};
+                if constexpr (NOUT == 2) val[1] = {rc_.re, rc_.im};
             } else {
                 val[0] = {rc_.re, rc_.im};
             }
@@ -1416,7 +1428,7 @@ constexpr int max_support() {
     return sizeof(T) == 4 ? 8 : 16;
 }
 
-template <class T, int NOUT>
+template <class T, int NOUT, bool FORM>
 void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
                  const Rec<T>* recs, const std::uint32_t* off, const T* coeff, const DevIn& in,
                  int G, int ngroups, int TvP, std::uint32_t rc, const std::uint32_t* item_off,
@@ -1424,9 +1436,9 @@ void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo&
 #define GRID_CASE(SS)                                                                           \
     case SS:                                                                                    \
         if constexpr (SS <= max_support<T>()) {                                                 \
-            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT>,                        \
+            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT, FORM>,                  \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
-            k_grid<T, SS, NOUT><<<nblocks, grid_threads<T>(), lds, st>>>(g, recs, off, coeff, in, G,  \
+            k_grid<T, SS, NOUT, FORM><<<nblocks, grid_threads<T>(), lds, st>>>(g, recs, off, coeff, in, G,  \
                                                                      ngroups, TvP, rc, item_off, \
                                                                      order, out);               \
         } else {                                                                                \
@@ -1584,7 +1596,7 @@ std::vector<std::uint32_t> tile_order(const Geo& g) {
 
 template <class T>
 void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st, HostKeep& keep,
-                 Tables<T>& tb) {
+                 Tables<T>& tb, int oset) {
     const std::vector<double> key = table_key<T>(G, g);
     const bool cached = ws.table_key == key;
     tb.twu = static_cast<cx<T>*>(ws.ptr[kSlotTwU]);
@@ -1595,11 +1607,12 @@ void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st,
     tb.screen = static_cast<double*>(ws.ptr[kSlotScreen]);
     tb.corr = static_cast<double*>(ws.ptr[kSlotCorr]);
     const std::vector<int> okey{g.Tu, g.Tv, g.ntu, g.ntv};
-    if (ws.order_key != okey) {
-        tb.order = upload_vec<std::uint32_t>(ws, kSlotOrder, tile_order(g), st, keep.u);
-        ws.order_key = okey;
+    const int oslot = oset ? kSlotOrder2 : kSlotOrder;
+    if (ws.order_key[oset] != okey) {
+        upload_vec<std::uint32_t>(ws, oslot, tile_order(g), st, keep.u);
+        ws.order_key[oset] = okey;
     }
-    tb.order = static_cast<std::uint32_t*>(ws.ptr[kSlotOrder]);
+    tb.order = static_cast<std::uint32_t*>(ws.ptr[oslot]);
     if (cached) {
         tb.du = ws.du;
         tb.dv = ws.dv;
@@ -1633,6 +1646,14 @@ void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st,
     ws.du = tb.du;
     ws.dv = tb.dv;
     ws.table_key = key;
+}
+
+// Compute units of the current device.
+int device_cus() {
+    int dev = 0, n = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(n, 1);
 }
 
 // One count from the device (the stream is synchronised).
@@ -1678,7 +1699,9 @@ template <class T>
 constexpr std::size_t grid_lds() {
     return 160 * 1024 - kGridBatch * sizeof(Rec<T>) - 2 * 10 * 16 * sizeof(T) - 256;
 }
-constexpr std::uint32_t kGridChunk = 1024;       // visibilities per gridding item
+// Visibilities per gridding item. Larger items were measured slower on the
+// 464M-visibility workload (4096: same, 16384: 1.35x, 65536: 2.6x the time).
+constexpr std::uint32_t kGridChunk = 1024;
 constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
 constexpr std::size_t kDegridLds = 64 * 1024;   // staged planes per workgroup
 
@@ -1744,22 +1767,26 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
     return true;
 }
 
+// The active visibilities (activity of `din`) as records in bucket order, in
+// the slots of record set `set` (0 or 1); the records' values are formed
+// from `vals` (din when null).
 template <class T, int MODE, bool USE_VIS>
 void sort_records(Workspace& ws, const DevIn& din, const Geo& g, long long nvis, hipStream_t st,
-                  Rec<T>** recs_out, std::uint32_t** off_out) {
+                  Rec<T>** recs_out, std::uint32_t** off_out, int set = 0,
+                  const DevIn* vals = nullptr) {
     const long long nkeys = (long long)g.ntu * g.ntv * 4 * g.nb;
-    std::uint32_t* keys = static_cast<std::uint32_t*>(ws.get(kSlotKeys, sizeof(std::uint32_t) * g.n));
-    std::uint32_t* counts = static_cast<std::uint32_t*>(ws.get(kSlotCounts, sizeof(std::uint32_t) * (nkeys + 1)));
-    std::uint32_t* off = static_cast<std::uint32_t*>(ws.get(kSlotOffsets, sizeof(std::uint32_t) * (nkeys + 1)));
-    std::uint32_t* cursor = static_cast<std::uint32_t*>(ws.get(kSlotCursor, sizeof(std::uint32_t) * (nkeys + 1)));
-    Rec<T>* recs = static_cast<Rec<T>*>(ws.get(kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
+    std::uint32_t* keys = static_cast<std::uint32_t*>(ws.get(set ? kSlotKeys2 : kSlotKeys, sizeof(std::uint32_t) * g.n));
+    std::uint32_t* counts = static_cast<std::uint32_t*>(ws.get(set ? kSlotCounts2 : kSlotCounts, sizeof(std::uint32_t) * (nkeys + 1)));
+    std::uint32_t* off = static_cast<std::uint32_t*>(ws.get(set ? kSlotOffsets2 : kSlotOffsets, sizeof(std::uint32_t) * (nkeys + 1)));
+    std::uint32_t* cursor = static_cast<std::uint32_t*>(ws.get(set ? kSlotCursor2 : kSlotCursor, sizeof(std::uint32_t) * (nkeys + 1)));
+    Rec<T>* recs = static_cast<Rec<T>*>(ws.get(set ? kSlotRecs2 : kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
     HIPCHK(hipMemsetAsync(counts, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
     HIPCHK(hipMemsetAsync(cursor, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
     const int eblocks = int((g.n + 255) / 256);
     k_keys<T, USE_VIS><<<eblocks, 256, 0, st>>>(din, g, keys, counts);
     HIPCHK(hipGetLastError());
     exclusive_scan(ws, counts, off, nkeys + 1, st);
-    k_scatter<T, MODE><<<eblocks, 256, 0, st>>>(din, g, keys, off, cursor, recs);
+    k_scatter<T, MODE><<<eblocks, 256, 0, st>>>(vals ? *vals : din, g, keys, off, cursor, recs);
     HIPCHK(hipGetLastError());
     *recs_out = recs;
     *off_out = off;
@@ -1840,6 +1867,81 @@ void set_lds(K kernel, std::size_t bytes) {
 // Column block of the u transforms: as many columns as fit 8192 values.
 int col_block(int nu) { return std::max(1, std::min(8, 8192 / nu)); }
 
+
+// ------------------------------------------------------------ prepared calls
+
+// Partial sums of a block of 256 threads into part[blockIdx.x].
+__device__ __forceinline__ void block_sum(double v, double* part) {
+    __shared__ double s[256];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+constexpr int kRedBlocks = 512;
+
+// The residual visibilities r = data - pred (pred null: the data) and the
+// misfit partials sum w |r|^2, formed in double from the values in T
+// (wgridder.rs:3318-3335, 3337-3353).
+template <class T>
+__global__ __launch_bounds__(256) void k_residual(DevIn in, const cx<T>* pred, cx<T>* r, long long n,
+                                                  double* part) {
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        cx<T> d = vis_get<T>(in, i);
+        if (pred) d = csub(d, pred[i]);
+        r[i] = d;
+        T w = T(1);
+        wgt_get<T>(in, i, w);
+        const double re = double(d.re), im = double(d.im);
+        q += double(w) * (re * re + im * im);
+    }
+    block_sum(q, part);
+}
+
+// vis[i] *= w[i].
+template <class T>
+__global__ __launch_bounds__(256) void k_weigh(DevIn in, cx<T>* vis, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        T w = T(1);
+        if (wgt_get<T>(in, i, w)) vis[i] = {vis[i].re * w, vis[i].im * w};
+    }
+}
+
+// Partials of sum x y in double.
+template <class T>
+__global__ __launch_bounds__(256) void k_dot(const T* x, const T* y, long long n, double* part) {
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        q += double(x[i]) * double(y[i]);
+    block_sum(q, part);
+}
+
+// x += alpha p, r -= alpha ap, and the partials of |r|^2 (clean.rs:291-296).
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_step(T* x, T* r, const T* p, const T* ap, double alpha,
+                                                 long long n, double* part) {
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        x[i] = T(double(x[i]) + alpha * double(p[i]));
+        const T ri = T(double(r[i]) - alpha * double(ap[i]));
+        r[i] = ri;
+        q += double(ri) * double(ri);
+    }
+    block_sum(q, part);
+}
+
+// p = r + beta p (clean.rs:298-300).
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_dir(T* p, const T* r, double beta, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        p[i] = T(double(r[i]) + beta * double(p[i]));
+}
+
 }  // namespace
 
 void set_stage_timing(bool on) { g_timing = on; }
@@ -1848,6 +1950,80 @@ const char* stage_timing_name(int i) {
     return i >= 0 && i < int(g_times.size()) ? g_times[i].first.c_str() : "";
 }
 double stage_timing_ms(int i) { return i >= 0 && i < int(g_times.size()) ? g_times[i].second : 0.0; }
+
+// vis2dirty from sorted records: tables, gridding, the transforms and the
+// sum over planes into `dirty` and/or `psf` (nx * ny each). With `form` the
+// records hold unit values and the data values are formed from din.vis while
+// gridding (one output).
+template <class T>
+void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& din,
+                   const Rec<T>* recs, const std::uint32_t* off, int nout, bool form, T* dirty,
+                   T* psf, hipStream_t st, HostKeep& keep, StageTimer& timer) {
+    const long long npix = (long long)g.nx * g.ny;
+    Tables<T> tb;
+    make_tables<T>(ws, G, g, st, keep, tb, 0);
+    timer.mark("tables");
+    const int P = g.wgrid ? g.nplanes : 1;
+    const long long plane = (long long)g.nu * g.nv;
+    cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
+    // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
+    const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
+    const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
+    if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
+    const int G_planes = std::max(1, std::min(P, int(grid_lds<T>() / per_plane)));
+    const int ngroups = (P + G_planes - 1) / G_planes;
+    const int ntg = g.ntu * g.ntv * ngroups;
+    const std::size_t lds = per_plane * std::min(P, G_planes);
+    std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
+    std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
+    const std::uint32_t rc = kGridChunk;
+    k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, rc, items);
+    HIPCHK(hipGetLastError());
+    exclusive_scan(ws, items, item_off, ntg + 1, st);
+    if (nout == 2)
+        k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
+    else
+        k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
+    HIPCHK(hipGetLastError());
+    const long long ub = read_u32(ws, item_off + ntg, st);
+    if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
+    if (nout == 2)
+        launch_grid<T, 2, false>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
+                                 ngroups, TvP, rc, item_off, tb.order, grid);
+    else if (form)
+        launch_grid<T, 1, true>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
+                                ngroups, TvP, rc, item_off, tb.order, grid);
+    else
+        launch_grid<T, 1, false>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
+                                 ngroups, TvP, rc, item_off, tb.order, grid);
+    timer.mark("grid");
+    cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
+    T* proj = static_cast<T*>(ws.get(kSlotProj, sizeof(T) * (long long)P * npix));
+    const int C = col_block(g.nu);
+    T* outs[2] = {dirty ? dirty : psf, psf};
+    for (int oi = 0; oi < nout; ++oi) {
+        const cx<T>* gsrc = grid + (long long)oi * P * plane;
+        fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+            set_lds(k_rows_inv<T, NT, MT>, lds_v);
+            k_rows_inv<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+        });
+        HIPCHK(hipGetLastError());
+        timer.mark("fft_v");
+        fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+            set_lds(k_cols_inv<T, NT, MT>, lds_u);
+            k_cols_inv<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+        });
+        HIPCHK(hipGetLastError());
+        timer.mark("fft_u");
+        k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
+        HIPCHK(hipGetLastError());
+        timer.mark("sum");
+    }
+}
 
 template <class T>
 int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
@@ -1889,65 +2065,7 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         else
             sort_records<T, 1, false>(ws, din, g, (long long)nvis, st, &recs, &off);
         timer.mark("sort");
-        Tables<T> tb;
-        make_tables<T>(ws, G, g, st, keep, tb);
-        timer.mark("tables");
-        const int P = g.wgrid ? g.nplanes : 1;
-        const long long plane = (long long)g.nu * g.nv;
-        cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
-        // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
-        const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
-        const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
-        if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
-        const int G_planes = std::max(1, std::min(P, int(grid_lds<T>() / per_plane)));
-        const int ngroups = (P + G_planes - 1) / G_planes;
-        const int ntg = g.ntu * g.ntv * ngroups;
-        const std::size_t lds = per_plane * std::min(P, G_planes);
-        std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
-        std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
-        k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, kGridChunk, items);
-        HIPCHK(hipGetLastError());
-        exclusive_scan(ws, items, item_off, ntg + 1, st);
-        if (nout == 2)
-            k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
-        else
-            k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
-        HIPCHK(hipGetLastError());
-        const long long ub = read_u32(ws, item_off + ntg, st);
-        if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
-        if (nout == 2)
-            launch_grid<T, 2>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes, ngroups,
-                              TvP, kGridChunk, item_off, tb.order, grid);
-        else
-            launch_grid<T, 1>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes, ngroups,
-                              TvP, kGridChunk, item_off, tb.order, grid);
-        timer.mark("grid");
-        cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
-        T* proj = static_cast<T*>(ws.get(kSlotProj, sizeof(T) * (long long)P * npix));
-        const int C = col_block(g.nu);
-        T* outs[2] = {dirty ? dirty : psf, psf};
-        for (int oi = 0; oi < nout; ++oi) {
-            const cx<T>* gsrc = grid + (long long)oi * P * plane;
-            fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds_v = sizeof(cx<T>) * g.nv;
-                set_lds(k_rows_inv<T, NT, MT>, lds_v);
-                k_rows_inv<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
-            });
-            HIPCHK(hipGetLastError());
-            timer.mark("fft_v");
-            fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-                set_lds(k_cols_inv<T, NT, MT>, lds_u);
-                k_cols_inv<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
-            });
-            HIPCHK(hipGetLastError());
-            timer.mark("fft_u");
-            k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
-            HIPCHK(hipGetLastError());
-            timer.mark("sum");
-        }
+        grid_pipeline<T>(ws, G, g, din, recs, off, nout, false, dirty, psf, st, keep, timer);
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
         std::copy(G.info, G.info + kNInfo, info);
@@ -1956,6 +2074,58 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         err = e.msg;
         return 2;
     }
+}
+
+// dirty2vis from sorted records: the corrected image, the transforms and the
+// degridding into `cvis` (zeroed by the caller; records' values multiply).
+template <class T>
+void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T>* recs,
+                     const std::uint32_t* off, const T* dirty, cx<T>* cvis, hipStream_t st,
+                     HostKeep& keep, StageTimer& timer) {
+    Tables<T> tb;
+    make_tables<T>(ws, G, g, st, keep, tb, 1);
+    timer.mark("tables");
+    const int P = g.wgrid ? g.nplanes : 1;
+    const long long plane = (long long)g.nu * g.nv;
+    const long long npix = (long long)g.nx * g.ny;
+    T* img = static_cast<T*>(ws.get(kSlotImg, sizeof(T) * npix));
+    k_correct<T><<<(npix + 255) / 256, 256, 0, st>>>(dirty, img, g, tb.corr, tb.cfu, tb.cfv);
+    HIPCHK(hipGetLastError());
+    cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
+    cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
+    const int C = col_block(g.nu);
+    fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+        set_lds(k_cols_fwd<T, NT, MT>, lds_u);
+        k_cols_fwd<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+    });
+    HIPCHK(hipGetLastError());
+    timer.mark("fft_u");
+    fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+        set_lds(k_rows_fwd<T, NT, MT>, lds_v);
+        k_rows_fwd<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
+    });
+    HIPCHK(hipGetLastError());
+    timer.mark("fft_v");
+    const int nt = g.ntu * g.ntv;
+    std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
+    std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
+    k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, kDegridChunk, items);
+    HIPCHK(hipGetLastError());
+    exclusive_scan(ws, items, item_off, nt + 1, st);
+    const long long ub = read_u32(ws, item_off + nt, st);
+    if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
+    const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
+    if (dl <= kDegridLds)
+        launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, kDegridChunk,
+                               item_off, tb.order, cvis);
+    else
+        launch_degrid<T, false>(g.supp, int(ub), 0, st, g, recs, off, tb.coeff, grid, kDegridChunk,
+                                item_off, tb.order, cvis);
+    timer.mark("degrid");
 }
 
 template <class T>
@@ -1990,51 +2160,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         std::uint32_t* off;
         sort_records<T, 2, false>(ws, din, g, (long long)nvis, st, &recs, &off);
         timer.mark("sort");
-        Tables<T> tb;
-        make_tables<T>(ws, G, g, st, keep, tb);
-        timer.mark("tables");
-        const int P = g.wgrid ? g.nplanes : 1;
-        const long long plane = (long long)g.nu * g.nv;
-        const std::size_t npix = o.nx * o.ny;
-        T* img = static_cast<T*>(ws.get(kSlotImg, sizeof(T) * npix));
-        k_correct<T><<<(npix + 255) / 256, 256, 0, st>>>(dirty, img, g, tb.corr, tb.cfu, tb.cfv);
-        HIPCHK(hipGetLastError());
-        cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
-        cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
-        const int C = col_block(g.nu);
-        fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-            set_lds(k_cols_fwd<T, NT, MT>, lds_u);
-            k_cols_fwd<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
-        });
-        HIPCHK(hipGetLastError());
-        timer.mark("fft_u");
-        fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds_v = sizeof(cx<T>) * g.nv;
-            set_lds(k_rows_fwd<T, NT, MT>, lds_v);
-            k_rows_fwd<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
-        });
-        HIPCHK(hipGetLastError());
-        timer.mark("fft_v");
-        const int nt = g.ntu * g.ntv;
-        std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
-        std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
-        k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, kDegridChunk, items);
-        HIPCHK(hipGetLastError());
-        exclusive_scan(ws, items, item_off, nt + 1, st);
-        const long long ub = read_u32(ws, item_off + nt, st);
-        if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
-        const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
-        cx<T>* cvis = reinterpret_cast<cx<T>*>(vis);
-        if (dl <= kDegridLds)
-            launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, kDegridChunk,
-                                   item_off, tb.order, cvis);
-        else
-            launch_degrid<T, false>(g.supp, int(ub), 0, st, g, recs, off, tb.coeff, grid, kDegridChunk,
-                                    item_off, tb.order, cvis);
-        timer.mark("degrid");
+        degrid_pipeline<T>(ws, G, g, recs, off, dirty, reinterpret_cast<cx<T>*>(vis), st, keep, timer);
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
         std::copy(G.info, G.info + kNInfo, info);
@@ -2044,6 +2170,333 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         return 2;
     }
 }
+
+
+// ------------------------------------------------------------ prepared plans
+
+// The plan and sorted layouts of a fixed set of visibilities kept between
+// calls (core/src/wgridder.rs:3165-3499): the scan, the kernel choice and
+// both sorts are done once; the active set is the nonzero weights. Owns a
+// work space of its own and device copies of uvw and the weights.
+template <class T>
+struct PreparedImpl final : PreparedBase {
+    int device = 0;
+    Workspace ws;
+    Options o;
+    Geometry G;
+    Geo gg{}, gd{};
+    long long n = 0, nvis = 0;
+    std::size_t nrow = 0, nchan = 0;
+    std::vector<double> freq;
+    double* uvw = nullptr;
+    void* wgt = nullptr;
+    std::size_t wbytes = 0;
+    int wkind = kWgtNone;
+    Rec<T>* rg = nullptr;
+    std::uint32_t* og = nullptr;
+    Rec<T>* rd = nullptr;
+    std::uint32_t* od = nullptr;
+    double* foc = nullptr;
+
+    ~PreparedImpl() override {
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+        for (int k = 0; k < kSlotCount; ++k)
+            if (ws.ptr[k]) (void)hipFree(ws.ptr[k]);
+        if (ws.pinned) (void)hipHostFree(ws.pinned);
+        if (uvw) (void)hipFree(uvw);
+        if (wgt) (void)hipFree(wgt);
+        if (cur != device) (void)hipSetDevice(cur);
+    }
+    bool single() const override { return sizeof(T) == 4; }
+    std::size_t active() const override { return std::size_t(nvis); }
+    std::size_t memory() const override {
+        std::size_t b = sizeof(double) * 3 * nrow + wbytes;
+        for (int k = 0; k < kSlotCount; ++k) b += ws.size[k];
+        return b;
+    }
+    void release_grids() override {
+        std::lock_guard<std::mutex> lk(ws.mu);
+        for (int k : {int(kSlotGrid), int(kSlotTmp), int(kSlotProj), int(kSlotImg), int(kSlotVbuf), int(kSlotSolve)}) {
+            if (ws.ptr[k]) (void)hipFree(ws.ptr[k]);
+            ws.ptr[k] = nullptr;
+            ws.size[k] = 0;
+        }
+    }
+    DevIn din(const void* vis, int vis_kind) const {
+        return DevIn{uvw, foc, vis, wgt, nullptr, nullptr, vis_kind, wkind};
+    }
+};
+
+namespace {
+
+template <class T>
+double sum_partials(Workspace& ws, const double* part, int nblk, hipStream_t st) {
+    double* h = static_cast<double*>(ws.host(sizeof(double) * nblk));
+    HIPCHK(hipMemcpyAsync(h, part, sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double q = 0.0;
+    for (int b = 0; b < nblk; ++b) q += h[b];
+    return q;
+}
+
+int red_blocks(long long n) {
+    return int(std::max<long long>(1, std::min<long long>(kRedBlocks, (n + 255) / 256)));
+}
+
+// vis = A dirty (no weights) at the active visibilities, zero elsewhere.
+template <class T>
+void prepared_predict(PreparedImpl<T>& p, const T* image, cx<T>* vis, hipStream_t st, HostKeep& keep,
+                      StageTimer& timer) {
+    HIPCHK(hipMemsetAsync(vis, 0, sizeof(cx<T>) * p.n, st));
+    if (p.nvis == 0) return;
+    degrid_pipeline<T>(p.ws, p.G, p.gd, p.rd, p.od, image, vis, st, keep, timer);
+}
+
+// image = A^H W vis over the active visibilities.
+template <class T>
+void prepared_image(PreparedImpl<T>& p, const void* vis, int vis_kind, T* image, hipStream_t st,
+                    HostKeep& keep, StageTimer& timer) {
+    if (p.nvis == 0) {
+        HIPCHK(hipMemsetAsync(image, 0, sizeof(T) * p.o.nx * p.o.ny, st));
+        return;
+    }
+    grid_pipeline<T>(p.ws, p.G, p.gg, p.din(vis, vis_kind), p.rg, p.og, 1, true, image, nullptr, st,
+                     keep, timer);
+}
+
+constexpr int vis_kind_of(float) { return kVisC64; }
+constexpr int vis_kind_of(double) { return kVisC128; }
+
+}  // namespace
+
+template <class T>
+PreparedBase* prepared_new(const Options& o, const DevInputs& in, const double* freq_host,
+                           hipStream_t st, int& status, std::string& err) {
+    status = 1;
+    if (!common_checks(o, in, freq_host, err)) return nullptr;
+    if (in.wgt_kind != kWgtNone && in.wgt_kind != kWgtF32 && in.wgt_kind != kWgtF64) {
+        err = "prepared weights must be float32 or float64";
+        return nullptr;
+    }
+    status = 2;
+    std::unique_ptr<PreparedImpl<T>> p(new PreparedImpl<T>());
+    try {
+        p->device = device_of();
+        p->o = o;
+        p->nrow = in.nrow;
+        p->nchan = in.nchan;
+        p->n = (long long)(in.nrow * in.nchan);
+        p->freq.assign(freq_host, freq_host + in.nchan);
+        HIPCHK(hipMalloc(&p->uvw, sizeof(double) * 3 * std::max<std::size_t>(in.nrow, 1)));
+        HIPCHK(hipMemcpyAsync(p->uvw, in.uvw, sizeof(double) * 3 * in.nrow, hipMemcpyDeviceToDevice, st));
+        p->wkind = in.wgt_kind;
+        if (in.wgt_kind != kWgtNone) {
+            p->wbytes = std::size_t(p->n) * (in.wgt_kind == kWgtF32 ? 4 : 8);
+            HIPCHK(hipMalloc(&p->wgt, std::max<std::size_t>(p->wbytes, 8)));
+            HIPCHK(hipMemcpyAsync(p->wgt, in.wgt, p->wbytes, hipMemcpyDeviceToDevice, st));
+        }
+        HostKeep keep;
+        p->foc = upload_foc(p->ws, freq_host, in.nchan, st, keep);
+        DevIn d = p->din(nullptr, kVisUnit);
+        Geo g = base_geo(in.nrow, in.nchan, o);
+        double wlo = 0, whi = 0;
+        p->nvis = (long long)run_scan<T, false>(p->ws, d, g, st, wlo, whi);
+        if (p->nvis > 0) {
+            if (!make_geometry(o, sizeof(T) == 4, std::size_t(p->nvis), wlo, whi, p->G, err)) {
+                status = 1;
+                return nullptr;
+            }
+            p->gg = g;
+            p->gd = g;
+            if (!fill_geo<T>(p->G, p->gg, 1, false, err) || !fill_geo<T>(p->G, p->gd, 1, true, err)) {
+                status = 1;
+                return nullptr;
+            }
+            // gridding: unit values with the weights; degridding: the shift
+            // phase alone (the prediction carries no weights)
+            sort_records<T, 1, false>(p->ws, d, p->gg, p->nvis, st, &p->rg, &p->og, 0);
+            DevIn unweighted = d;
+            unweighted.wgt = nullptr;
+            unweighted.wgt_kind = kWgtNone;
+            sort_records<T, 2, false>(p->ws, d, p->gd, p->nvis, st, &p->rd, &p->od, 1, &unweighted);
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        status = 0;
+        return p.release();
+    } catch (const DevError& e) {
+        err = e.msg;
+        return nullptr;
+    }
+}
+
+template <class T>
+int prepared_vis2dirty(PreparedBase* base, const void* vis, int vis_kind, T* dirty, double* info,
+                       hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        prepared_image<T>(p, vis, vis_kind, dirty, st, keep, timer);
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        std::fill(info, info + kNInfo, 0.0);
+        if (p.nvis) std::copy(p.G.info, p.G.info + kNInfo, info);
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template <class T>
+int prepared_dirty2vis(PreparedBase* base, const T* dirty, bool weighted, T* vis, hipStream_t st,
+                       std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        cx<T>* cv = reinterpret_cast<cx<T>*>(vis);
+        prepared_predict<T>(p, dirty, cv, st, keep, timer);
+        if (weighted && p.wkind != kWgtNone && p.n > 0)
+            k_weigh<T><<<red_blocks(p.n), 256, 0, st>>>(p.din(nullptr, kVisUnit), cv, p.n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template <class T>
+int prepared_residual(PreparedBase* base, const void* vis, int vis_kind, const T* model, T* dirty,
+                      double* misfit, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        cx<T>* r = static_cast<cx<T>*>(p.ws.get(kSlotVbuf, sizeof(cx<T>) * std::max<long long>(p.n, 1)));
+        if (model) prepared_predict<T>(p, model, r, st, keep, timer);
+        const int nblk = red_blocks(p.n);
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+        k_residual<T><<<nblk, 256, 0, st>>>(p.din(vis, vis_kind), model ? r : nullptr, r, p.n, part);
+        HIPCHK(hipGetLastError());
+        *misfit = sum_partials<T>(p.ws, part, nblk, st);
+        timer.mark("residual");
+        prepared_image<T>(p, r, vis_kind_of(T()), dirty, st, keep, timer);
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+namespace {
+
+// out = A^H W A x.
+template <class T>
+void gram_apply(PreparedImpl<T>& p, const T* x, T* out, hipStream_t st, HostKeep& keep, StageTimer& timer) {
+    cx<T>* v = static_cast<cx<T>*>(p.ws.get(kSlotVbuf, sizeof(cx<T>) * std::max<long long>(p.n, 1)));
+    prepared_predict<T>(p, x, v, st, keep, timer);
+    prepared_image<T>(p, v, vis_kind_of(T()), out, st, keep, timer);
+}
+
+template <class T>
+double dot(PreparedImpl<T>& p, const T* x, const T* y, long long n, hipStream_t st) {
+    double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+    const int nblk = red_blocks(n);
+    k_dot<T><<<nblk, 256, 0, st>>>(x, y, n, part);
+    HIPCHK(hipGetLastError());
+    return sum_partials<T>(p.ws, part, nblk, st);
+}
+
+}  // namespace
+
+template <class T>
+int prepared_gram(PreparedBase* base, const T* x, T* out, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        gram_apply<T>(p, x, out, st, keep, timer);
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+// Conjugate gradients for A^H W A x = rhs from zero (core/src/clean.rs:263-306):
+// stops after `iterations` Gram applications, when |r| <= tolerance |rhs|,
+// or when <p, A p> <= 0. Vectors in T, inner products in double.
+template <class T>
+int prepared_cg(PreparedBase* base, const T* rhs, T* x, int iterations, double tolerance, int* used,
+                double* rel, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        const long long m = (long long)p.o.nx * p.o.ny;
+        T* work = static_cast<T*>(p.ws.get(kSlotSolve, sizeof(T) * 3 * std::max<long long>(m, 1)));
+        T *r = work, *dir = work + m, *ap = work + 2 * m;
+        const double norm_d = std::sqrt(dot<T>(p, rhs, rhs, m, st));
+        HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * m, st));
+        HIPCHK(hipMemcpyAsync(r, rhs, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(dir, rhs, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        double rr = norm_d * norm_d;
+        int k = 0;
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+        const int nblk = red_blocks(m);
+        while (k < iterations && std::sqrt(rr) > tolerance * norm_d) {
+            gram_apply<T>(p, dir, ap, st, keep, timer);
+            const double pap = dot<T>(p, dir, ap, m, st);
+            if (!(pap > 0.0)) break;
+            const double alpha = rr / pap;
+            k_cg_step<T><<<nblk, 256, 0, st>>>(x, r, dir, ap, alpha, m, part);
+            HIPCHK(hipGetLastError());
+            const double rr_next = sum_partials<T>(p.ws, part, nblk, st);
+            const double beta = rr_next / rr;
+            k_cg_dir<T><<<nblk, 256, 0, st>>>(dir, r, beta, m);
+            HIPCHK(hipGetLastError());
+            rr = rr_next;
+            ++k;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        *used = k;
+        *rel = std::sqrt(rr) / norm_d;
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+#define RANG_PREPARED_INST(T)                                                                     \
+    template PreparedBase* prepared_new<T>(const Options&, const DevInputs&, const double*,         \
+                                           hipStream_t, int&, std::string&);                        \
+    template int prepared_vis2dirty<T>(PreparedBase*, const void*, int, T*, double*, hipStream_t,  \
+                                       std::string&);                                               \
+    template int prepared_dirty2vis<T>(PreparedBase*, const T*, bool, T*, hipStream_t, std::string&); \
+    template int prepared_residual<T>(PreparedBase*, const void*, int, const T*, T*, double*,       \
+                                      hipStream_t, std::string&);                                   \
+    template int prepared_gram<T>(PreparedBase*, const T*, T*, hipStream_t, std::string&);         \
+    template int prepared_cg<T>(PreparedBase*, const T*, T*, int, double, int*, double*, hipStream_t, \
+                                std::string&);
+RANG_PREPARED_INST(float)
+RANG_PREPARED_INST(double)
+#undef RANG_PREPARED_INST
 
 template int device_vis2dirty<float>(const Options&, const DevInputs&, const double*, float*,
                                      float*, double*, hipStream_t, std::string&);

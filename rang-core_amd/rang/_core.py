@@ -76,6 +76,22 @@ SIGNATURES = {
         [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_dirty2vis_dev_f32": (
         [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepare_dev_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp], _vp),
+    "rang_amd_wgridder_prepare_dev_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp], _vp),
+    "rang_amd_wgridder_prepared_vis2dirty_f64": ([_vp, _vp, _i32, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_vis2dirty_f32": ([_vp, _vp, _i32, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_dirty2vis_f64": ([_vp, _vp, _i32, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_dirty2vis_f32": ([_vp, _vp, _i32, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_residual_f64": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_residual_f32": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_gram_f64": ([_vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_gram_f32": ([_vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_cg_f64": ([_vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_cg_f32": ([_vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_active": ([_vp], _sz),
+    "rang_amd_wgridder_prepared_memory": ([_vp], _sz),
+    "rang_amd_wgridder_prepared_release_grids": ([_vp], None),
+    "rang_amd_wgridder_prepared_free": ([_vp], None),
     "rang_amd_stage_timing": ([_i32], None),
     "rang_amd_stage_count": ([], _i32),
     "rang_amd_stage_name": ([_i32], ctypes.c_char_p),

@@ -1,0 +1,116 @@
+"""Prepared plans on the GPU: the plan and sorted layouts of a fixed set of
+visibilities kept between calls, as rang-core's `wgridder::Prepared`
+(core/src/wgridder.rs:3165-3499), on torch tensors in HBM.
+
+    p = Prepared(opts, uvw, freq, wgt, single=True)
+    dirty = p.vis2dirty(vis)            # image of vis with the prepared weights
+    vis = p.dirty2vis(model)            # A model (weighted=True: times the weights)
+    dirty, q = p.residual(vis, model)   # image of vis - A model, sum w |vis - A model|^2
+    out = p.gram(x)                     # A^H W A x
+    x, used, rel = p.conjugate_gradients(rhs, iterations, tolerance)
+
+The active set is the nonzero weights. All arrays are contiguous tensors on
+the current GPU; calls run on torch's current stream.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _core
+from .device import _VIS_KIND, _WGT_KIND, _check, _ptr, _stream
+
+
+class Prepared:
+    def __init__(self, opts, uvw, freq, wgt=None, *, single):
+        o = _core.options(opts)
+        self.nx, self.ny = o.nx, o.ny
+        self.nrow, self.nchan = int(uvw.shape[0]), int(freq.shape[0])
+        self.single = bool(single)
+        self.real = torch.float32 if single else torch.float64
+        self.complex = torch.complex64 if single else torch.complex128
+        self.device = uvw.device
+        lib = _core.library()
+        fn = lib.rang_amd_wgridder_prepare_dev_f32 if single else lib.rang_amd_wgridder_prepare_dev_f64
+        kind = _core.WGT_NONE if wgt is None else _WGT_KIND[wgt.dtype]
+        if wgt is not None and wgt.numel() != self.nrow * self.nchan:
+            raise ValueError("array lengths disagree")
+        h = fn(ctypes.byref(o), self.nrow, self.nchan, _ptr(uvw), _ptr(freq), _ptr(wgt), kind, _stream())
+        if not h:
+            raise ValueError(_core.wgridder_last_error())
+        self._h = ctypes.c_void_p(h)
+        self._suf = "f32" if single else "f64"
+
+    def _fn(self, name):
+        return getattr(_core.library(), f"rang_amd_wgridder_prepared_{name}_{self._suf}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _core.library().rang_amd_wgridder_prepared_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _image(self):
+        return torch.empty((self.nx, self.ny), dtype=self.real, device=self.device)
+
+    def _vis(self, vis):
+        if vis.numel() != self.nrow * self.nchan:
+            raise ValueError("array lengths disagree")
+        return _VIS_KIND[vis.dtype]
+
+    def active(self):
+        """Active visibilities (nonzero weight)."""
+        return int(_core.library().rang_amd_wgridder_prepared_active(self._h))
+
+    def memory(self):
+        """Device bytes the plan holds."""
+        return int(_core.library().rang_amd_wgridder_prepared_memory(self._h))
+
+    def release_grids(self):
+        """Free the work grids; the next call allocates them again."""
+        _core.library().rang_amd_wgridder_prepared_release_grids(self._h)
+
+    def vis2dirty(self, vis, out=None):
+        kind = self._vis(vis)
+        d = out if out is not None else self._image()
+        info = np.zeros(_core.N_INFO)
+        _check(self._fn("vis2dirty")(self._h, _ptr(vis), kind, _ptr(d),
+                                      ctypes.c_void_p(info.ctypes.data), _stream()))
+        return d
+
+    def dirty2vis(self, dirty, weighted=False, out=None):
+        v = out if out is not None else torch.empty((self.nrow, self.nchan), dtype=self.complex,
+                                                    device=self.device)
+        _check(self._fn("dirty2vis")(self._h, _ptr(dirty), int(bool(weighted)), _ptr(v), _stream()))
+        return v
+
+    def residual(self, vis, model=None, out=None):
+        """(image of vis - A model, sum w |vis - A model|^2)."""
+        kind = self._vis(vis)
+        d = out if out is not None else self._image()
+        q = ctypes.c_double(0.0)
+        _check(self._fn("residual")(self._h, _ptr(vis), kind, _ptr(model), _ptr(d), ctypes.byref(q),
+                                     _stream()))
+        return d, q.value
+
+    def gram(self, x, out=None):
+        """A^H W A x."""
+        d = out if out is not None else self._image()
+        _check(self._fn("gram")(self._h, _ptr(x), _ptr(d), _stream()))
+        return d
+
+    def conjugate_gradients(self, rhs, iterations, tolerance, out=None):
+        """CG on A^H W A x = rhs from zero (core/src/clean.rs:263-306):
+        (x, Gram applications used, |rhs - A^H W A x| / |rhs|)."""
+        x = out if out is not None else self._image()
+        used = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        _check(self._fn("cg")(self._h, _ptr(rhs), _ptr(x), int(iterations), float(tolerance),
+                               ctypes.byref(used), ctypes.byref(rel), _stream()))
+        return x, used.value, rel.value

@@ -20,7 +20,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 20
+    assert len(names) == 36
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
