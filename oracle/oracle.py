@@ -308,3 +308,61 @@ def dft_vis(uvw, freq, image, wgt, dx, dy, rows=None, divide_by_n=False, **flips
     lib().or_dft_vis(u.size, _p(u), _p(v), _p(w), l.size, _p(l), _p(m), _p(img),
                      int(divide_by_n), _p(wt), _p(out))
     return out.reshape(nrow, nchan) if rows is None else out
+
+
+# ---------------------------------------------------------------- w-stacking
+
+
+class WsInputs(ctypes.Structure):
+    """core/src/ffi_grid.rs:7-29."""
+    _fields_ = [(n, ctypes.c_size_t) for n in ("nrow", "nchan", "width", "height", "kernel_size",
+                                                "oversampling", "n_w_layers")] + [
+        ("pixel_size_x", ctypes.c_double), ("pixel_size_y", ctypes.c_double),
+        ("padding", ctypes.c_double), ("uvw", ctypes.c_void_p), ("frequencies", ctypes.c_void_p),
+        ("data", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("imaging_weights", ctypes.c_void_p)]
+
+
+class WsOutputs(ctypes.Structure):
+    _fields_ = [("dirty", ctypes.c_void_p), ("psf", ctypes.c_void_p), ("stats", ctypes.c_void_p)]
+
+
+WS_STAT_NAMES = ("image_weight", "gridded_count", "visibility_weight_sum", "max_gridded_weight",
+                 "min_w", "max_w", "psf_normalization", "n_w_layers")
+
+
+def ws_kernel(kernel_size=7, oversampling=1023):
+    out = np.zeros(kernel_size * oversampling)
+    if lib().ws_kernel(ctypes.c_size_t(kernel_size), ctypes.c_size_t(oversampling), _p(out)):
+        raise ValueError("invalid kernel size")
+    return out
+
+
+def ws_fft2_backward(grid):
+    """The oracle's unnormalised backward 2-D DFT of a (height, width) complex
+    array (rows of width, then columns)."""
+    g = np.ascontiguousarray(grid, np.complex128).copy()
+    lib().ws_fft2_backward(_p(g), ctypes.c_size_t(g.shape[1]), ctypes.c_size_t(g.shape[0]))
+    return g
+
+
+def ws_invert(uvw, frequencies, data, weights, imaging_weights, *, width, height, pixel_size,
+              w_layers, kernel_size=7, oversampling=1023, padding=1.2):
+    """The oracle's w-stacking inversion: (height, width) dirty and PSF in
+    FITS order and the eight counters."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    frequencies = np.ascontiguousarray(frequencies, np.float64)
+    data = np.ascontiguousarray(data, np.complex64)
+    weights = np.ascontiguousarray(weights, np.float32)
+    imaging_weights = np.ascontiguousarray(imaging_weights, np.float32)
+    nrow, nchan = data.shape
+    inp = WsInputs(nrow, nchan, width, height, kernel_size, oversampling, w_layers, pixel_size,
+                   pixel_size, padding, uvw.ctypes.data, frequencies.ctypes.data, data.ctypes.data,
+                   weights.ctypes.data, imaging_weights.ctypes.data)
+    dirty = np.zeros((height, width))
+    psf = np.zeros((height, width))
+    stats = np.zeros(8)
+    out = WsOutputs(dirty.ctypes.data, psf.ctypes.data, stats.ctypes.data)
+    if lib().ws_invert(ctypes.byref(inp), ctypes.byref(out)):
+        raise ValueError("invalid w-stacking inputs")
+    rec = dict(zip(WS_STAT_NAMES, stats.tolist()))
+    return {"dirty": dirty, "psf": psf, "stats": rec}

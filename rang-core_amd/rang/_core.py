@@ -49,6 +49,26 @@ class WgridderOptions(ctypes.Structure):
 # Every symbol of include/rang_core_wgridder.h with its ctypes signature.
 _vp, _sz, _i32, _dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_double
 _OPT = ctypes.POINTER(WgridderOptions)
+
+
+class GridInputs(ctypes.Structure):
+    """`GridInputs` (core/src/ffi_grid.rs:7-29)."""
+
+    _fields_ = [(n, ctypes.c_size_t) for n in ("nrow", "nchan", "width", "height", "kernel_size",
+                                                "oversampling", "n_w_layers")] + [
+        ("pixel_size_x", ctypes.c_double), ("pixel_size_y", ctypes.c_double),
+        ("padding", ctypes.c_double), ("uvw", ctypes.c_void_p), ("frequencies", ctypes.c_void_p),
+        ("data", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("imaging_weights", ctypes.c_void_p)]
+
+
+class GridOutputs(ctypes.Structure):
+    """`GridOutputs` (core/src/ffi_grid.rs:31-45)."""
+
+    _fields_ = [("dirty", ctypes.c_void_p), ("psf", ctypes.c_void_p), ("stats", ctypes.c_void_p)]
+
+
+GRID_N_STATS = 8
+GRID_N_TIMINGS = 14
 SIGNATURES = {
     "rang_core_wgridder_vis2dirty_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "rang_core_wgridder_vis2dirty_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
@@ -92,6 +112,12 @@ SIGNATURES = {
     "rang_amd_wgridder_prepared_memory": ([_vp], _sz),
     "rang_amd_wgridder_prepared_release_grids": ([_vp], None),
     "rang_amd_wgridder_prepared_free": ([_vp], None),
+    "rang_core_grid_invert": ([ctypes.POINTER(GridInputs), ctypes.POINTER(GridOutputs)], _i32),
+    "rang_core_grid_invert_gpu": (
+        [ctypes.POINTER(GridInputs), ctypes.POINTER(GridOutputs), _i32, _vp], _i32),
+    "rang_core_grid_kernel": ([_sz, _sz, _vp], _i32),
+    "rang_amd_grid_last_error": ([_vp, _sz], _sz),
+    "rang_amd_device_name": ([_vp, _sz], _sz),
     "rang_amd_stage_timing": ([_i32], None),
     "rang_amd_stage_count": ([], _i32),
     "rang_amd_stage_name": ([_i32], ctypes.c_char_p),
@@ -280,3 +306,82 @@ def wgridder_last_error():
     buf = ctypes.create_string_buffer(max(int(n), 1))
     lib.rang_core_wgridder_last_error(ctypes.cast(buf, ctypes.c_void_p), n)
     return buf.raw[:n].decode("utf-8", "replace")
+
+
+# ---------------------------------------------------------------- w-stacking
+# The grid_* functions of rang._core (core/py/src/lib.rs:906-1049).
+
+
+def _grid_structs(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers, pixel_size_x,
+                  pixel_size_y, padding, uvw, frequencies, data, weights, imaging_weights, dirty, psf,
+                  stats):
+    cells, pixels = nrow * nchan, width * height
+    uvw = _checked(uvw, np.float64, "uvw")
+    frequencies = _checked(frequencies, np.float64, "frequencies")
+    data = _checked(data, np.float32, "data")
+    weights = _checked(weights, np.float32, "weights")
+    imaging_weights = _checked(imaging_weights, np.float32, "imaging_weights")
+    dirty = _checked(dirty, np.float64, "dirty")
+    psf = _checked(psf, np.float64, "psf")
+    stats = _checked(stats, np.float64, "stats")
+    if (uvw.size != 3 * nrow or frequencies.size != nchan or data.size != 2 * cells
+            or weights.size != cells or imaging_weights.size != cells or dirty.size != pixels
+            or psf.size != pixels or stats.size != GRID_N_STATS):
+        raise ValueError("gridder array sizes disagree")
+    inp = GridInputs(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers,
+                     pixel_size_x, pixel_size_y, padding, uvw.ctypes.data, frequencies.ctypes.data,
+                     data.ctypes.data, weights.ctypes.data, imaging_weights.ctypes.data)
+    out = GridOutputs(dirty.ctypes.data, psf.ctypes.data, stats.ctypes.data)
+    return inp, out
+
+
+def grid_invert(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers, pixel_size_x,
+                pixel_size_y, padding, uvw, frequencies, data, weights, imaging_weights, dirty, psf,
+                stats):
+    """`rang_core_grid_invert` into caller-owned arrays; returns its status."""
+    inp, out = _grid_structs(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers,
+                             pixel_size_x, pixel_size_y, padding, uvw, frequencies, data, weights,
+                             imaging_weights, dirty, psf, stats)
+    return int(library().rang_core_grid_invert(ctypes.byref(inp), ctypes.byref(out)))
+
+
+def grid_invert_gpu(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers, pixel_size_x,
+                    pixel_size_y, padding, uvw, frequencies, data, weights, imaging_weights,
+                    precision, dirty, psf, stats, timings):
+    """`rang_core_grid_invert_gpu`; `precision` "f64" or "f32"."""
+    if precision not in ("f64", "f32"):
+        raise ValueError(f"precision must be 'f64' or 'f32', not {precision!r}")
+    timings = _checked(timings, np.float64, "timings")
+    if timings.size != GRID_N_TIMINGS:
+        raise ValueError("gridder array sizes disagree")
+    inp, out = _grid_structs(nrow, nchan, width, height, kernel_size, oversampling, n_w_layers,
+                             pixel_size_x, pixel_size_y, padding, uvw, frequencies, data, weights,
+                             imaging_weights, dirty, psf, stats)
+    return int(library().rang_core_grid_invert_gpu(ctypes.byref(inp), ctypes.byref(out),
+                                                   0 if precision == "f64" else 1, _ptr(timings)))
+
+
+def grid_kernel(kernel_size, oversampling, kernel):
+    """`rang_core_grid_kernel` into a float64 array of kernel_size * oversampling."""
+    kernel = _checked(kernel, np.float64, "kernel")
+    if kernel.size != kernel_size * oversampling:
+        return 1
+    return int(library().rang_core_grid_kernel(int(kernel_size), int(oversampling), _ptr(kernel)))
+
+
+def grid_last_error():
+    lib = library()
+    n = lib.rang_amd_grid_last_error(None, 0)
+    buf = ctypes.create_string_buffer(max(int(n), 1))
+    lib.rang_amd_grid_last_error(ctypes.cast(buf, ctypes.c_void_p), n)
+    return buf.raw[:n].decode("utf-8", "replace")
+
+
+def grid_gpu_adapter():
+    """The GPU the library runs on (core/py/src/lib.rs:1028-1030)."""
+    lib = library()
+    buf = ctypes.create_string_buffer(256)
+    n = lib.rang_amd_device_name(ctypes.cast(buf, ctypes.c_void_p), 255)
+    if n == 0:
+        raise RuntimeError("no HIP device")
+    return buf.raw[:min(int(n), 255)].decode("utf-8", "replace")

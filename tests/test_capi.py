@@ -10,17 +10,17 @@ from conftest import ROOT, ref_case
 from oracle import oracle as O
 from rang import _core, core_wgridder
 
-HEADER = ROOT / "include" / "rang_core_wgridder.h"
+HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h"]
 
 
 def declared():
-    text = HEADER.read_text()
+    text = "".join(h.read_text() for h in HEADERS)
     return sorted(set(re.findall(r"\b(rang_(?:core|amd)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 36
+    assert len(names) == 41
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
@@ -67,3 +67,13 @@ def test_host_side_errors_match_the_reference():
     with pytest.raises(ValueError, match="18 gridder options"):
         _core.wgridder_vis2dirty_f64([1.0] * 17, uvw, freq, vis, wgt, None, None,
                                      np.empty((nx, ny)), None, np.empty(14))
+
+
+def test_grid_kernel_and_padding_without_a_gpu():
+    from rang import core_grid
+
+    assert np.array_equal(core_grid.kernel(7, 1023), O.ws_kernel(7, 1023))
+    # core/src/grid.rs:954-970
+    assert (core_grid.padded_size(512), core_grid.padded_size(100)) == (616, 120)
+    with pytest.raises(RuntimeError, match="code 1"):
+        core_grid.kernel(0, 1023)

@@ -1,0 +1,129 @@
+"""The HIP w-stacking gridder (core/src/grid.rs through the C ABI of
+core/src/ffi_grid.rs and core/src/gpu/ffi.rs) against rang-core's golden
+vectors and the oracle.
+
+Tolerances (max |difference| over the image peak): 1e-12 in double (the
+GPU adds each layer's samples in a different order and transforms with its
+own FFT), 2e-5 in single precision (grid and transforms in float, sums in
+double). Counters that are sums of float32 weights in double: relative
+1e-12; the gridded count and the w range: exact."""
+
+import numpy as np
+import pytest
+from conftest import ROOT
+from ws_cases import golden_case, point_source_case, survey_case
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = ROOT / "tests" / "golden" / "ws_golden.npz"
+TOL = {"f64": 1e-12, "f32": 2e-5}
+
+
+def gpu(case, precision="f64", device="gpu", **kw):
+    from rang import core_grid
+
+    return core_grid.invert(case["uvw"], case["frequencies"], case["data"], case["weights"],
+                            case["imaging_weights"], width=case["width"], height=case["height"],
+                            scale_arcsec=case["pixel_size"] / core_grid.ARCSEC,
+                            w_layers=case["w_layers"], device=device, precision=precision, **kw)
+
+
+def oracle(case, **kw):
+    return O.ws_invert(case["uvw"], case["frequencies"], case["data"], case["weights"],
+                       case["imaging_weights"], width=case["width"], height=case["height"],
+                       pixel_size=case["pixel_size"], w_layers=case["w_layers"], **kw)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def same_stats(got, ref, precision="f64", exact_sums=False):
+    for k in ("gridded_count", "min_w", "max_w", "n_w_layers"):
+        assert got[k] == ref[k], k
+    for k in ("image_weight", "visibility_weight_sum", "max_gridded_weight"):
+        tol = 0.0 if exact_sums else 1e-12
+        assert abs(got[k] - ref[k]) <= tol * abs(ref[k]), k
+    # the PSF centre comes out of the grid and transforms: their precision
+    assert abs(got["psf_normalization"] - ref["psf_normalization"]) <= TOL[precision] * ref["psf_normalization"]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_golden_case(precision):
+    z = np.load(GOLD, allow_pickle=False)
+    c = golden_case()
+    r = gpu(c, precision)
+    assert rel(r["dirty"], z["dirty"]) < TOL[precision]
+    assert rel(r["psf"], z["psf"]) < TOL[precision]
+    names = O.WS_STAT_NAMES
+    ref = dict(zip(names, z["stats"].tolist()))
+    same_stats(r["stats"], ref, precision, exact_sums=True)
+    assert r["gpu"]["batches"] >= 1 and r["gpu"]["gpu_bytes"] > 0
+
+
+def test_cpu_entry_is_the_double_path():
+    z = np.load(GOLD, allow_pickle=False)
+    r = gpu(golden_case(), device="cpu")
+    assert rel(r["dirty"], z["dirty"]) < TOL["f64"]
+    assert r["gridder"] == "wstack"
+
+
+def test_kernel_through_the_abi():
+    from rang import core_grid
+
+    z = np.load(GOLD, allow_pickle=False)
+    assert np.abs(core_grid.kernel(7, 1023) - z["kernel"]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_point_source_lands_at_wsclean_pixel(precision):
+    # core/src/grid.rs:1036-1063
+    r = gpu(point_source_case(4), precision)
+    d = r["dirty"]
+    y, x = np.unravel_index(np.argmax(d), d.shape)
+    assert (x, y) == (16 - 5, 16 - 3)
+    assert abs(r["psf"][16, 16] - 1.0) <= 2 * np.finfo(float).eps
+    assert abs(d.max() - 1.0) < 0.05
+    assert r["stats"]["gridded_count"] == 108
+
+
+@pytest.mark.parametrize("layers", [1, 8, 40])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_matches_oracle(layers, precision):
+    c = survey_case(seed=layers, w_layers=layers)
+    r, ref = gpu(c, precision), oracle(c)
+    assert rel(r["dirty"], ref["dirty"]) < TOL[precision]
+    assert rel(r["psf"], ref["psf"]) < TOL[precision]
+    same_stats(r["stats"], ref["stats"], precision)
+
+
+def test_many_layers_uneven_imaging_weights():
+    # core/src/grid.rs:1065-1094's case: 40 layers, imaging weights 0.5..2.5
+    c = point_source_case(40, imaging=(0.5 + np.arange(120) % 3).reshape(60, 2))
+    r, ref = gpu(c), oracle(c)
+    assert rel(r["dirty"], ref["dirty"]) < TOL["f64"]
+    assert rel(r["psf"], ref["psf"]) < TOL["f64"]
+    same_stats(r["stats"], ref["stats"])
+
+
+def test_errors():
+    from rang import core_grid
+
+    c = golden_case()
+    # no weight at all: invalid (1 from the CPU entry, 3 from the GPU entry)
+    z = dict(c, weights=np.zeros_like(c["weights"]))
+    with pytest.raises(RuntimeError, match="code 1"):
+        gpu(z, device="cpu")
+    with pytest.raises(RuntimeError, match="code 3"):
+        gpu(z)
+    # even kernel size
+    with pytest.raises(RuntimeError, match="code 1"):
+        gpu(c, device="cpu", kernel_size=6)
+    # a padded side with a prime factor above 11 (42 -> 52 = 4 x 13)
+    with pytest.raises(RuntimeError, match="products of 2, 3, 5, 7 and 11"):
+        gpu(dict(c, width=42), device="cpu")
+    with pytest.raises(ValueError):
+        gpu(c, precision="f16")
+    assert "gfx950" in __import__("rang._core", fromlist=["x"]).grid_gpu_adapter()
