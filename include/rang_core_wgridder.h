@@ -156,6 +156,28 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions *opts, size_t 
                                             const float *dirty, const void *wgt, int32_t wgt_kind,
                                             const uint8_t *mask, float *vis, double *info,
                                             void *stream);
+/* --- Plans without gridding: the reference's Rust-side wgridder::plan and
+ * gridding_footprint (core/src/wgridder.rs:3525-3586), which have no C entry
+ * point there, on host buffers. The scan runs on the GPU. ------------------
+ * plan: the 14 info doubles the call with these arrays would report (all zero
+ * with no active visibility); vis kind unit plans the PSF / degridding pass.
+ * footprint: the plan of the gridding pass on float64 weights for nout (1 or
+ * 2) images and the device bytes that pass holds in this implementation. */
+int32_t rang_amd_wgridder_plan_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                   const double *uvw, const double *freq, const void *vis,
+                                   int32_t vis_kind, const void *wgt, int32_t wgt_kind,
+                                   const uint8_t *mask, double *info);
+int32_t rang_amd_wgridder_plan_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                   const double *uvw, const double *freq, const void *vis,
+                                   int32_t vis_kind, const void *wgt, int32_t wgt_kind,
+                                   const uint8_t *mask, double *info);
+int32_t rang_amd_wgridder_footprint_f64(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                        const double *uvw, const double *freq, const double *wgt,
+                                        size_t nout, double *info, size_t *bytes);
+int32_t rang_amd_wgridder_footprint_f32(const WgridderOptions *opts, size_t nrow, size_t nchan,
+                                        const double *uvw, const double *freq, const double *wgt,
+                                        size_t nout, double *info, size_t *bytes);
+
 /* --- Prepared plans: the reference's Rust-side Prepared (core/src/wgridder.rs:
  * 3165-3499), which has no C entry point there, on device buffers. ---------
  * A plan and both sorted layouts of one set of visibilities (uvw nrow x 3,

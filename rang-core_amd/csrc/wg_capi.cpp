@@ -309,6 +309,44 @@ int32_t d2v_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan
 }
 
 
+// The plan of a call on host buffers, without gridding (wgridder::plan,
+// core/src/wgridder.rs:3525-3549; with nout > 0, gridding_footprint,
+// :3551-3586, on float64 weights).
+template <class T>
+int32_t plan_host(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan, const double* uvw,
+                  const double* freq, const void* vis, int32_t vis_kind, const void* wgt, int32_t wgt_kind,
+                  const std::uint8_t* mask, int nout, double* info, std::size_t* bytes) {
+    try {
+        if (!opts) return fail(1, "null options");
+        if (!uvw || !freq || !info || (nout > 0 && !bytes)) return fail(1, "null array");
+        std::string err;
+        if (!kinds_ok(vis, vis_kind, wgt, nullptr, wgt_kind, err)) return fail(1, err);
+        const Options o = to_options(*opts);
+        if (!host_checks(o, nrow, nchan, freq, err)) return fail(1, err);
+        Staging* s = staging();
+        if (!s) return fail(2, "no HIP device");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const std::size_t n = nrow * nchan;
+        Stage st{s};
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = st.up(0, uvw, sizeof(double) * 3 * nrow);
+        in.freq = st.up(1, freq, sizeof(double) * nchan);
+        in.vis = st.up(2, static_cast<const char*>(vis), vis_bytes(vis_kind) * n);
+        in.vis_kind = vis_kind;
+        in.wgt = st.up(3, static_cast<const char*>(wgt), wgt_bytes(wgt_kind) * n);
+        in.wgt_kind = wgt_kind;
+        in.mask = st.up(5, mask, n);
+        if (!st.ok) return fail(2, st.err);
+        const int32_t status = device_plan<T>(o, in, freq, vis_kind != kVisUnit, nout, info, bytes,
+                                              s->stream, err);
+        return status ? fail(status, err) : 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
 template <class T>
 RangPrepared* prepare_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
                           const double* uvw, const double* freq, const void* wgt, int32_t wgt_kind,
@@ -574,6 +612,27 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions* opts, size_t 
 RANG_PREPARED_ABI(f64, double)
 RANG_PREPARED_ABI(f32, float)
 #undef RANG_PREPARED_ABI
+
+#define RANG_PLAN_ABI(SUF, T)                                                                       \
+    int32_t rang_amd_wgridder_plan_##SUF(const WgridderOptions* opts, size_t nrow, size_t nchan,       \
+                                         const double* uvw, const double* freq, const void* vis,       \
+                                         int32_t vis_kind, const void* wgt, int32_t wgt_kind,          \
+                                         const uint8_t* mask, double* info) {                          \
+        return plan_host<T>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt_kind, mask, 0, info, \
+                            nullptr);                                                                  \
+    }                                                                                                  \
+    int32_t rang_amd_wgridder_footprint_##SUF(const WgridderOptions* opts, size_t nrow, size_t nchan,  \
+                                              const double* uvw, const double* freq,                   \
+                                              const double* wgt, size_t nout, double* info,            \
+                                              size_t* bytes) {                                         \
+        if (!wgt) return fail(1, "null array");                                                        \
+        if (nout == 0 || nout > 2) return fail(1, "one or two images");                                \
+        return plan_host<T>(opts, nrow, nchan, uvw, freq, nullptr, kVisUnit, wgt, kWgtF64, nullptr,    \
+                            int(nout), info, bytes);                                                   \
+    }
+RANG_PLAN_ABI(f64, double)
+RANG_PLAN_ABI(f32, float)
+#undef RANG_PLAN_ABI
 
 size_t rang_amd_wgridder_prepared_active(const RangPrepared* h) {
     const PreparedBase* p = reinterpret_cast<const PreparedBase*>(h);

@@ -39,6 +39,12 @@ template <class T>
 int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
                      const T* dirty, T* vis, double* info, hipStream_t stream, std::string& err);
 
+// The plan without gridding (core/src/wgridder.rs:3525-3586); footprint: the
+// gridding pass's device bytes for nout images (null or nout 0: none).
+template <class T>
+int device_plan(const Options& o, const DevInputs& in, const double* freq_host, bool use_vis, int nout,
+                double* info, std::size_t* footprint, hipStream_t stream, std::string& err);
+
 // A prepared set of visibilities (core/src/wgridder.rs:3165-3499): plan and
 // sorted layouts kept on the device between calls.
 struct PreparedBase {

@@ -2172,6 +2172,51 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
 }
 
 
+// ------------------------------------------------------------ plans
+
+// The plan a call would choose, without gridding (core/src/wgridder.rs:
+// 3525-3549: plan; 3551-3586: gridding_footprint): the scan on the device,
+// then the kernel and grid choice. `footprint` (nout > 0) receives the
+// device bytes the gridding pass of this implementation would hold (grids,
+// column transforms and plane terms, sorted records).
+template <class T>
+int device_plan(const Options& o, const DevInputs& in, const double* freq_host, bool use_vis, int nout,
+                double* info, std::size_t* footprint, hipStream_t st, std::string& err) {
+    try {
+        if (!common_checks(o, in, freq_host, err)) return 1;
+        Workspace& ws = workspace(device_of());
+        std::lock_guard<std::mutex> lk(ws.mu);
+        HostKeep keep;
+        DevIn din{in.uvw, nullptr, in.vis, in.wgt, in.wgt2, in.mask, in.vis_kind, in.wgt_kind};
+        Geo g = base_geo(in.nrow, in.nchan, o);
+        din.foc = upload_foc(ws, freq_host, in.nchan, st, keep);
+        double wlo = 0, whi = 0;
+        const unsigned long long nvis = use_vis ? run_scan<T, true>(ws, din, g, st, wlo, whi)
+                                                : run_scan<T, false>(ws, din, g, st, wlo, whi);
+        std::fill(info, info + kNInfo, 0.0);
+        if (footprint) *footprint = 0;
+        if (nvis == 0) return 0;
+        Geometry G;
+        if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
+        std::copy(G.info, G.info + kNInfo, info);
+        if (footprint && nout > 0) {
+            const std::size_t P = G.do_wgridding ? std::max<std::size_t>(G.nplanes, 1) : 1;
+            const std::size_t plane = G.nu * G.nv * sizeof(cx<T>);
+            *footprint = P * nout * plane + P * G.nu * o.ny * sizeof(cx<T>) + P * o.nx * o.ny * sizeof(T) +
+                         std::size_t(nvis) * sizeof(Rec<T>);
+        }
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template int device_plan<float>(const Options&, const DevInputs&, const double*, bool, int, double*,
+                                std::size_t*, hipStream_t, std::string&);
+template int device_plan<double>(const Options&, const DevInputs&, const double*, bool, int, double*,
+                                 std::size_t*, hipStream_t, std::string&);
+
 // ------------------------------------------------------------ prepared plans
 
 // The plan and sorted layouts of a fixed set of visibilities kept between

@@ -96,6 +96,10 @@ SIGNATURES = {
         [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_dirty2vis_dev_f32": (
         [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_plan_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _vp], _i32),
+    "rang_amd_wgridder_plan_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _vp], _i32),
+    "rang_amd_wgridder_footprint_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
+    "rang_amd_wgridder_footprint_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
     "rang_amd_wgridder_prepare_dev_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp], _vp),
     "rang_amd_wgridder_prepare_dev_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp], _vp),
     "rang_amd_wgridder_prepared_vis2dirty_f64": ([_vp, _vp, _i32, _vp, _vp, _vp], _i32),

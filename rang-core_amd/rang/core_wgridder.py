@@ -197,6 +197,52 @@ def dirty2vis(*, uvw, freq, dirty, wgt=None, mask=None, pixsize_x, pixsize_y, ep
     return (vis, info_dict(info)) if return_info else vis
 
 
+def plan(*, uvw, freq, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, vis=None, wgt=None,
+         mask=None, single=False, **kw):
+    """The plan (info dict) a call on these arrays would choose, without
+    gridding (core/src/wgridder.rs:3525-3549); `vis` None plans the PSF or
+    degridding pass."""
+    import ctypes
+
+    uvw, freq, shape = _coords(uvw, freq)
+    wgt, mask = _weights(wgt, mask, shape)
+    vis_kind = _core.VIS_UNIT
+    if vis is not None:
+        vis = _visibilities(vis, shape)
+        vis_kind = _core.VIS_C64 if vis.dtype == np.complex64 else _core.VIS_C128
+    wgt_kind = _core.WGT_NONE if wgt is None else (
+        _core.WGT_F32 if wgt.dtype == np.float32 else _core.WGT_F64)
+    o = _core.options(_options(npix_x, npix_y, pixsize_x, pixsize_y, epsilon, kw))
+    info = np.zeros(N_INFO)
+    lib = _core.library()
+    fn = lib.rang_amd_wgridder_plan_f32 if single else lib.rang_amd_wgridder_plan_f64
+    p = lambda a: None if a is None else ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    _check(fn(ctypes.byref(o), shape[0], shape[1], p(uvw), p(freq), p(vis), vis_kind, p(wgt),
+              wgt_kind, p(mask), p(info)))
+    return info_dict(info)
+
+
+def gridding_footprint(*, uvw, freq, wgt, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, nout=1,
+                       single=False, **kw):
+    """(plan, bytes): the plan of the gridding pass on float64 weights and the
+    device bytes that pass holds (core/src/wgridder.rs:3551-3586)."""
+    import ctypes
+
+    uvw, freq, shape = _coords(uvw, freq)
+    wgt = np.ascontiguousarray(wgt, np.float64)
+    if wgt.shape != shape:
+        raise ValueError("array lengths disagree")
+    o = _core.options(_options(npix_x, npix_y, pixsize_x, pixsize_y, epsilon, kw))
+    info = np.zeros(N_INFO)
+    nbytes = ctypes.c_size_t(0)
+    lib = _core.library()
+    fn = lib.rang_amd_wgridder_footprint_f32 if single else lib.rang_amd_wgridder_footprint_f64
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    _check(fn(ctypes.byref(o), shape[0], shape[1], p(uvw), p(freq), p(wgt), int(nout), p(info),
+              ctypes.byref(nbytes)))
+    return info_dict(info), int(nbytes.value)
+
+
 def info_dict(info):
     rec = {k: float(v) for k, v in zip(INFO_NAMES, info)}
     for k in ("kernel_index", "supp", "nu", "nv", "nplanes", "nvis"):

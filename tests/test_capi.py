@@ -20,7 +20,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 41
+    assert len(names) == 45
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name

@@ -250,3 +250,28 @@ def test_fits_image_orientation():
     # flip_v; FITS order transposes
     peak = np.unravel_index(np.argmax(got["dirty"]), got["dirty"].shape)
     assert peak == (width // 2 + 4, width // 2 + 7)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_plan_and_footprint_without_gridding(single):
+    # core/src/wgridder.rs:3525-3586: the plan a call would choose
+    uvw, freq, vis, wgt, nx, ny, dx = ref_case(nrow=50, seed=8)
+    eps = 1e-5 if single else 1e-9
+    cdt, rdt = (np.complex64, np.float32) if single else (np.complex128, np.float64)
+    p = W.plan(uvw=uvw, freq=freq, vis=vis.astype(cdt), wgt=wgt.astype(rdt), npix_x=nx, npix_y=ny,
+               pixsize_x=dx, pixsize_y=dx, epsilon=eps, single=single)
+    _, info = W.vis2dirty(uvw=uvw, freq=freq, vis=vis.astype(cdt), wgt=wgt.astype(rdt), npix_x=nx,
+                          npix_y=ny, pixsize_x=dx, pixsize_y=dx, epsilon=eps, return_info=True)
+    assert p == info
+    pf, nbytes = W.gridding_footprint(uvw=uvw, freq=freq, wgt=wgt, npix_x=nx, npix_y=ny,
+                                      pixsize_x=dx, pixsize_y=dx, epsilon=eps, nout=2, single=single)
+    _, _, info2 = W.dirty_psf(uvw=uvw, freq=freq, vis=vis.astype(cdt), wgt=wgt.astype(rdt),
+                              npix_x=nx, npix_y=ny, pixsize_x=dx, pixsize_y=dx, epsilon=eps,
+                              single=single, return_info=True)
+    assert pf == info2
+    cx = 8 if single else 16
+    assert nbytes >= 2 * pf["nplanes"] * pf["nu"] * pf["nv"] * cx
+    # no active visibility: an empty plan
+    e = W.plan(uvw=uvw, freq=freq, vis=None, wgt=np.zeros_like(wgt), npix_x=nx, npix_y=ny,
+               pixsize_x=dx, pixsize_y=dx, epsilon=eps, single=single)
+    assert e["nvis"] == 0 and e["nu"] == 0
