@@ -50,6 +50,14 @@ void check(hipError_t e, const char* what) {
 
 #define HIPCHK(x) check((x), #x)
 
+// Blocks of a one-dimensional launch: a dispatch holds fewer than 2^32
+// work-items per grid dimension, so larger launches are refused here rather
+// than wrapped by the runtime.
+unsigned blocks_of(long long blocks, int threads) {
+    if (blocks <= 0 || blocks * threads >= (1LL << 32)) throw DevError{"launch above 2^32 work-items"};
+    return unsigned(blocks);
+}
+
 // ------------------------------------------------------------ workspace
 
 enum Slot : int {
@@ -88,6 +96,7 @@ enum Slot : int {
     kSlotVbuf,
     kSlotRed,
     kSlotSolve,
+    kSlotLong,  // twiddles of the transforms longer than one LDS
     kSlotCount
 };
 
@@ -189,7 +198,13 @@ struct Geo {
     int Tu, Tv, ntu, ntv;
     int tnx, tny;
     int divide_by_n;
+    // Transforms longer than one LDS along u or v: n = n1 n2 (0: in LDS).
+    // Their forward results sit at (k mod n1) n2 + k / n1.
+    int lu1, lu2, lv1, lv2;
 };
+
+__device__ __forceinline__ int upos(const Geo& g, int u) { return g.lu1 ? (u % g.lu1) * g.lu2 + u / g.lu1 : u; }
+__device__ __forceinline__ int vpos(const Geo& g, int v) { return g.lv1 ? (v % g.lv1) * g.lv2 + v / g.lv1 : v; }
 
 struct DevIn {
     const double* uvw;
@@ -1291,18 +1306,202 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
 template <class T, int NT, int MAXTOT>
 __global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
                                                           cx<T>* __restrict__ grid, FftDesc d,
-                                                          const cx<T>* __restrict__ tw, int ny) {
+                                                          const cx<T>* __restrict__ tw, Geo g) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
-    const int nv = d.n;
+    const int nv = d.n, ny = g.ny;
     const long long row = blockIdx.x;
+    const long long p = row / g.nu;
+    const long long src = p * g.nu + upos(g, int(row - p * g.nu));
     for (int e = threadIdx.x; e < nv; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
-    for (int j = threadIdx.x; j < ny; j += NT) buf[(nv - ny / 2 + j) % nv] = tmp[row * ny + j];
+    for (int j = threadIdx.x; j < ny; j += NT) buf[(nv - ny / 2 + j) % nv] = tmp[src * ny + j];
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, -1);
     cx<T>* dst = grid + row * nv;
     for (int e = threadIdx.x; e < nv; e += NT) dst[e] = buf[e];
+}
+
+// ------------------------------------------------------------ long transforms
+
+// Transforms longer than one LDS, n = n1 n2, in two passes over memory:
+// X[k1 + n1 k2] = sum_j2 w_n2^(j2 k2) w_n^(j2 k1) sum_j1 x[j1 n2 + j2] w_n1^(j1 k1)
+// (w_m = exp(sgn 2 pi i / m)). Pass 1 transforms along j1 and applies the
+// twiddle in place; pass 2 transforms along j2 and hands X[k1 + n1 k2] to
+// its consumer: in place at k1 n2 + k2, or gathered to the pruned output.
+// `twn` holds exp(-2 pi i k / n) for k < n (conjugated for sgn +1).
+
+template <class T>
+__device__ __forceinline__ cx<T> twn_at(const cx<T>* twn, int e, int sgn) {
+    cx<T> f = twn[e];
+    if (sgn > 0) f.im = -f.im;
+    return f;
+}
+
+// Rows of length n = n1 n2 at `data + row * n`; block: (row, C2 lines j2).
+template <class T, int NT, int MT>
+__global__ __launch_bounds__(NT) void k_lrow_p1(cx<T>* data, int n1, int n2, int C2, FftDesc d1,
+                                                const cx<T>* tw1, const cx<T>* twn, int sgn) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nb = (n2 + C2 - 1) / C2;
+    const long long row = blockIdx.x / nb;
+    const int j20 = int(blockIdx.x - row * nb) * C2, nc = min(C2, n2 - j20);
+    cx<T>* x = data + row * (long long)n1 * n2;
+    for (int e = threadIdx.x; e < n1 * nc; e += NT) {
+        const int j1 = e / nc, c = e - j1 * nc;
+        buf[c * n1 + j1] = x[(long long)j1 * n2 + j20 + c];
+    }
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d1, nc, tw1, sgn);
+    for (int e = threadIdx.x; e < n1 * nc; e += NT) {
+        const int k1 = e / nc, c = e - k1 * nc;
+        x[(long long)k1 * n2 + j20 + c] = cmul(buf[c * n1 + k1], twn_at(twn, (j20 + c) * k1, sgn));
+    }
+}
+
+// Pass 2 over rows; block: (row, K1 lines k1). MODE 0: in place; MODE 1: the
+// image columns j < ny of X[(n - ny/2 + j) mod n] into out (rows of ny).
+template <class T, int NT, int MT, int MODE>
+__global__ __launch_bounds__(NT) void k_lrow_p2(cx<T>* data, int n1, int n2, int K1, FftDesc d2,
+                                                const cx<T>* tw2, int sgn, cx<T>* out, int ny) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nb = (n1 + K1 - 1) / K1;
+    const long long row = blockIdx.x / nb;
+    const int k10 = int(blockIdx.x - row * nb) * K1, nc = min(K1, n1 - k10);
+    const int n = n1 * n2;
+    cx<T>* x = data + row * (long long)n;
+    for (int e = threadIdx.x; e < n2 * nc; e += NT) buf[e] = x[(long long)k10 * n2 + e];
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d2, nc, tw2, sgn);
+    for (int e = threadIdx.x; e < n2 * nc; e += NT) {
+        const int c = e / n2, k2 = e - c * n2;
+        if constexpr (MODE == 0) {
+            x[(long long)(k10 + c) * n2 + k2] = buf[e];
+        } else {
+            const int k = k10 + c + n1 * k2;
+            const int j = (k + ny / 2) % n;
+            if (j < ny) out[row * ny + j] = buf[e];
+        }
+    }
+}
+
+// Columns of [P][n][M] (lines (p, m), element j at (p n + j) M + m); block:
+// (p, Cm columns, C2 lines j2).
+template <class T, int NT, int MT>
+__global__ __launch_bounds__(NT) void k_lcol_p1(cx<T>* data, int n1, int n2, int M, int Cm, int C2,
+                                                FftDesc d1, const cx<T>* tw1, const cx<T>* twn, int sgn) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nmb = (M + Cm - 1) / Cm, njb = (n2 + C2 - 1) / C2;
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lin / (nmb * njb), rem = lin - p * nmb * njb;
+    const int mb = rem / njb, jb = rem - mb * njb;
+    const int m0 = mb * Cm, ncm = min(Cm, M - m0), j20 = jb * C2, nc2 = min(C2, n2 - j20);
+    const int lines = ncm * nc2;
+    cx<T>* x = data + (long long)p * n1 * n2 * M;
+    for (int e = threadIdx.x; e < n1 * lines; e += NT) {
+        const int j1 = e / lines, r = e - j1 * lines, c2 = r / ncm, cm = r - c2 * ncm;
+        buf[r * n1 + j1] = x[((long long)j1 * n2 + j20 + c2) * M + m0 + cm];
+    }
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d1, lines, tw1, sgn);
+    for (int e = threadIdx.x; e < n1 * lines; e += NT) {
+        const int k1 = e / lines, r = e - k1 * lines, c2 = r / ncm, cm = r - c2 * ncm;
+        x[((long long)k1 * n2 + j20 + c2) * M + m0 + cm] = cmul(buf[r * n1 + k1], twn_at(twn, (j20 + c2) * k1, sgn));
+    }
+}
+
+// Pass 2 over columns; block: (p, Cm columns, K1 lines k1). MODE 0: in place;
+// MODE 2: the image rows i < nx of X[(n - nx/2 + i) mod n] projected with the
+// plane's w-screen into proj (as k_cols_inv).
+template <class T, int NT, int MT, int MODE>
+__global__ __launch_bounds__(NT) void k_lcol_p2(cx<T>* data, int n1, int n2, int M, int Cm, int K1,
+                                                FftDesc d2, const cx<T>* tw2, int sgn, T* proj, Geo g,
+                                                const double* screen) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int nmb = (M + Cm - 1) / Cm, nkb = (n1 + K1 - 1) / K1;
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lin / (nmb * nkb), rem = lin - p * nmb * nkb;
+    const int mb = rem / nkb, kb = rem - mb * nkb;
+    const int m0 = mb * Cm, ncm = min(Cm, M - m0), k10 = kb * K1, nc1 = min(K1, n1 - k10);
+    const int lines = ncm * nc1, n = n1 * n2;
+    cx<T>* x = data + (long long)p * n * M;
+    for (int e = threadIdx.x; e < n2 * lines; e += NT) {
+        const int j2 = e / lines, r = e - j2 * lines, c1 = r / ncm, cm = r - c1 * ncm;
+        buf[r * n2 + j2] = x[((long long)(k10 + c1) * n2 + j2) * M + m0 + cm];
+    }
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d2, lines, tw2, sgn);
+    const double w0 = g.wmin + double(p) * g.dw;
+    for (int e = threadIdx.x; e < n2 * lines; e += NT) {
+        const int k2 = e / lines, r = e - k2 * lines, c1 = r / ncm, cm = r - c1 * ncm;
+        const cx<T> v = buf[r * n2 + k2];
+        if constexpr (MODE == 0) {
+            x[((long long)(k10 + c1) * n2 + k2) * M + m0 + cm] = v;
+        } else {
+            const int k = k10 + c1 + n1 * k2;
+            const int i = (k + g.nx / 2) % n;
+            if (i < g.nx) {
+                const int j = m0 + cm;
+                T term;
+                if (g.wgrid) {
+                    const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], true);
+                    term = v.re * ph.re - v.im * ph.im;
+                } else {
+                    term = v.re;
+                }
+                proj[(long long)p * g.nx * g.ny + (long long)i * g.ny + j] = term;
+            }
+        }
+    }
+}
+
+// d2v with long u: the image rows times the plane's w-screen placed on the
+// grid rows of tmp ([P][nu][ny], zeros elsewhere), as k_cols_fwd places them.
+// One workgroup per grid row (p, u): a flat launch over P nu ny values
+// would pass the 2^32 work-items one dispatch dimension holds.
+template <class T>
+__global__ __launch_bounds__(256) void k_place_cols(const T* __restrict__ img, cx<T>* tmp, Geo g,
+                                                    const double* __restrict__ screen) {
+    const long long row = blockIdx.x;
+    const int p = int(row / g.nu), u = int(row - (long long)p * g.nu);
+    const int i = (u + g.nx / 2) % g.nu;
+    cx<T>* dst = tmp + row * g.ny;
+    if (i >= g.nx) {
+        for (int j = threadIdx.x; j < g.ny; j += 256) dst[j] = cx<T>{T(0), T(0)};
+        return;
+    }
+    const double w0 = g.wmin + double(p) * g.dw;
+    const T* src = img + (long long)i * g.ny;
+    for (int j = threadIdx.x; j < g.ny; j += 256) {
+        const T v = src[j];
+        cx<T> val;
+        if (g.wgrid) {
+            const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], false);
+            val = {v * ph.re, v * ph.im};
+        } else {
+            val = {v, T(0)};
+        }
+        dst[j] = val;
+    }
+}
+
+// d2v with long v: grid rows from the image columns of tmp (its rows at their
+// physical place), zeros elsewhere, as k_rows_fwd places them. One workgroup
+// per grid row.
+template <class T>
+__global__ __launch_bounds__(256) void k_place_rows(const cx<T>* __restrict__ tmp, cx<T>* grid, Geo g) {
+    const long long row = blockIdx.x;
+    const long long p = row / g.nu;
+    const cx<T>* src = tmp + (p * g.nu + upos(g, int(row - p * g.nu))) * g.ny;
+    cx<T>* dst = grid + row * g.nv;
+    for (int v = threadIdx.x; v < g.nv; v += 256) {
+        const int j = (v + g.ny / 2) % g.nv;
+        dst[v] = j < g.ny ? src[j] : cx<T>{T(0), T(0)};
+    }
 }
 
 // Work items of the degridding: per tile rank (`order`, densest first) its
@@ -1359,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict_
             const int i = rem / SV, j = rem - (rem / SV) * SV;
             const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
             const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
-            buf[e] = grid[(long long)p * plane + (long long)grow * g.nv + gcol];
+            buf[e] = grid[(long long)p * plane + (long long)grow * g.nv + vpos(g, gcol)];
         }
     }
     __syncthreads();
@@ -1376,7 +1575,7 @@ __global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict_
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 rows[k] = (tu * g.Tu + ou + k + g.nu / 2) % g.nu;
-                cols[k] = (tv * g.Tv + ov + k + g.nv / 2) % g.nv;
+                cols[k] = vpos(g, (tv * g.Tv + ov + k + g.nv / 2) % g.nv);
             }
         }
         const T fl = (rc_.idx & kFlip) ? T(-1) : T(1);
@@ -1705,6 +1904,35 @@ constexpr std::uint32_t kGridChunk = 1024;
 constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
 constexpr std::size_t kDegridLds = 64 * 1024;   // staged planes per workgroup
 
+// Longest transform done in one LDS pass (fft_maxtot); longer ones take two
+// passes over memory. RANG_FFT_LDS_MAX lowers the limit (a tuning and test
+// knob; it never raises it).
+template <class T>
+int lds_fft_limit() {
+    int lim = fft_maxtot<T>();
+    if (const char* e = std::getenv("RANG_FFT_LDS_MAX")) {
+        const long v = std::atol(e);
+        if (v >= 16 && v < lim) lim = int(v);
+    }
+    return lim;
+}
+
+// n = n1 n2 with both factors at most lim, n1 nearest sqrt(n).
+bool long_split(int n, int lim, int& n1, int& n2) {
+    n1 = n2 = 0;
+    double best = 1e300;
+    for (int d = 2; d <= lim && d < n; ++d) {
+        if (n % d || n / d > lim) continue;
+        const double dist = std::fabs(double(d) - std::sqrt(double(n)));
+        if (dist < best) {
+            best = dist;
+            n1 = d;
+            n2 = n / d;
+        }
+    }
+    return n1 > 0;
+}
+
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits kDegridLds.
@@ -1760,8 +1988,14 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
         err = "grid too large for one call";
         return false;
     }
-    if (std::max(g.nu, g.nv) > fft_maxtot<T>()) {
-        err = "grid dimension above the in-LDS transform limit of this build";
+    g.lu1 = g.lu2 = g.lv1 = g.lv2 = 0;
+    const int lim = lds_fft_limit<T>();
+    if (g.nu > lim && !long_split(g.nu, lim, g.lu1, g.lu2)) {
+        err = "no two-pass split of the grid size for the transforms";
+        return false;
+    }
+    if (g.nv > lim && !long_split(g.nv, lim, g.lv1, g.lv2)) {
+        err = "no two-pass split of the grid size for the transforms";
         return false;
     }
     return true;
@@ -1867,6 +2101,117 @@ void set_lds(K kernel, std::size_t bytes) {
 // Column block of the u transforms: as many columns as fit 8192 values.
 int col_block(int nu) { return std::max(1, std::min(8, 8192 / nu)); }
 
+// Twiddles of the two-pass transforms of one call.
+template <class T>
+struct LongTabs {
+    FftDesc u1, u2, v1, v2;
+    cx<T>*tu1 = nullptr, *tu2 = nullptr, *tun = nullptr, *tv1 = nullptr, *tv2 = nullptr, *tvn = nullptr;
+};
+
+template <class T>
+void make_long(Workspace& ws, const Geo& g, hipStream_t st, HostKeep& keep, LongTabs<T>& lt) {
+    std::vector<double> all;
+    long long ofs[6] = {0, 0, 0, 0, 0, 0};
+    auto add = [&](int k, const std::vector<double>& v) {
+        ofs[k] = (long long)all.size() / 2;
+        all.insert(all.end(), v.begin(), v.end());
+    };
+    auto roots = [](int n) {
+        std::vector<double> r(2 * std::size_t(n));
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k < n; ++k) {
+            const long double a = -2.0L * pi * (long double)k / (long double)n;
+            r[2 * k] = double(cosl(a));
+            r[2 * k + 1] = double(sinl(a));
+        }
+        return r;
+    };
+    std::vector<double> t;
+    if (g.lu1) {
+        if (!fft_describe(g.lu1, lt.u1, t)) throw DevError{"two-pass transform factor"};
+        add(0, t);
+        if (!fft_describe(g.lu2, lt.u2, t)) throw DevError{"two-pass transform factor"};
+        add(1, t);
+        add(2, roots(g.nu));
+    }
+    if (g.lv1) {
+        if (!fft_describe(g.lv1, lt.v1, t)) throw DevError{"two-pass transform factor"};
+        add(3, t);
+        if (!fft_describe(g.lv2, lt.v2, t)) throw DevError{"two-pass transform factor"};
+        add(4, t);
+        add(5, roots(g.nv));
+    }
+    if (all.empty()) return;
+    cx<T>* d = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotLong, all, st, keep));
+    if (g.lu1) {
+        lt.tu1 = d + ofs[0];
+        lt.tu2 = d + ofs[1];
+        lt.tun = d + ofs[2];
+    }
+    if (g.lv1) {
+        lt.tv1 = d + ofs[3];
+        lt.tv2 = d + ofs[4];
+        lt.tvn = d + ofs[5];
+    }
+}
+
+// Two-pass transforms of `rows` contiguous rows of length n1 n2 in place
+// (MODE 0) or, for MODE 1, with pass 2 gathering the ny image columns into out.
+template <class T, int MODE>
+void long_rows(cx<T>* data, long long rows, int n1, int n2, const FftDesc& d1, const FftDesc& d2,
+               const cx<T>* t1, const cx<T>* t2, const cx<T>* tn, int sgn, cx<T>* out, int ny,
+               hipStream_t st) {
+    const int budget = fft_maxtot<T>() / 2;
+    const int C2 = std::max(1, std::min(n2, budget / n1));
+    fft_dispatch<T>(n1 * C2, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds = sizeof(cx<T>) * n1 * C2;
+        set_lds(k_lrow_p1<T, NT, MT>, lds);
+        k_lrow_p1<T, NT, MT><<<blocks_of(rows * ((n2 + C2 - 1) / C2), NT), NT, lds, st>>>(data, n1, n2, C2, d1, t1, tn, sgn);
+    });
+    HIPCHK(hipGetLastError());
+    const int K1 = std::max(1, std::min(n1, budget / n2));
+    fft_dispatch<T>(n2 * K1, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds = sizeof(cx<T>) * n2 * K1;
+        set_lds(k_lrow_p2<T, NT, MT, MODE>, lds);
+        k_lrow_p2<T, NT, MT, MODE><<<blocks_of(rows * ((n1 + K1 - 1) / K1), NT), NT, lds, st>>>(data, n1, n2, K1, d2, t2, sgn,
+                                                                               out, ny);
+    });
+    HIPCHK(hipGetLastError());
+}
+
+// Two-pass transforms along the middle axis of [P][n1 n2][M] in place (MODE
+// 0) or, for MODE 2, with pass 2 projecting the image rows into proj.
+template <class T, int MODE>
+void long_cols(cx<T>* data, int P, int M, int n1, int n2, const FftDesc& d1, const FftDesc& d2,
+               const cx<T>* t1, const cx<T>* t2, const cx<T>* tn, int sgn, T* proj, const Geo& g,
+               const double* screen, hipStream_t st) {
+    const int budget = fft_maxtot<T>() / 2;
+    int Cm = std::max(1, std::min(std::min(16, M), budget / n1));
+    const int C2 = std::max(1, std::min(n2, budget / (n1 * Cm)));
+    long long nblk = (long long)P * ((M + Cm - 1) / Cm) * ((n2 + C2 - 1) / C2);
+    fft_dispatch<T>(n1 * Cm * C2, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds = sizeof(cx<T>) * n1 * Cm * C2;
+        set_lds(k_lcol_p1<T, NT, MT>, lds);
+        k_lcol_p1<T, NT, MT><<<blocks_of(nblk, NT), NT, lds, st>>>(data, n1, n2, M, Cm, C2, d1, t1, tn, sgn);
+    });
+    HIPCHK(hipGetLastError());
+    Cm = std::max(1, std::min(std::min(16, M), budget / n2));
+    const int K1 = std::max(1, std::min(n1, budget / (n2 * Cm)));
+    nblk = (long long)P * ((M + Cm - 1) / Cm) * ((n1 + K1 - 1) / K1);
+    fft_dispatch<T>(n2 * Cm * K1, [&](auto nt, auto mt) {
+        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+        const std::size_t lds = sizeof(cx<T>) * n2 * Cm * K1;
+        set_lds(k_lcol_p2<T, NT, MT, MODE>, lds);
+        k_lcol_p2<T, NT, MT, MODE><<<blocks_of(nblk, NT), NT, lds, st>>>(data, n1, n2, M, Cm, K1, d2, t2, sgn, proj, g,
+                                                              screen);
+    });
+    HIPCHK(hipGetLastError());
+}
+
+
 
 // ------------------------------------------------------------ prepared calls
 
@@ -1962,6 +2307,8 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     const long long npix = (long long)g.nx * g.ny;
     Tables<T> tb;
     make_tables<T>(ws, G, g, st, keep, tb, 0);
+    LongTabs<T> lt;
+    make_long<T>(ws, g, st, keep, lt);
     timer.mark("tables");
     const int P = g.wgrid ? g.nplanes : 1;
     const long long plane = (long long)g.nu * g.nv;
@@ -2002,22 +2349,32 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     const int C = col_block(g.nu);
     T* outs[2] = {dirty ? dirty : psf, psf};
     for (int oi = 0; oi < nout; ++oi) {
-        const cx<T>* gsrc = grid + (long long)oi * P * plane;
-        fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds_v = sizeof(cx<T>) * g.nv;
-            set_lds(k_rows_inv<T, NT, MT>, lds_v);
-            k_rows_inv<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
-        });
-        HIPCHK(hipGetLastError());
+        cx<T>* gsrc = grid + (long long)oi * P * plane;
+        if (g.lv1) {
+            long_rows<T, 1>(gsrc, (long long)P * g.nu, g.lv1, g.lv2, lt.v1, lt.v2, lt.tv1, lt.tv2, lt.tvn, +1,
+                            tmp, g.ny, st);
+        } else {
+            fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+                set_lds(k_rows_inv<T, NT, MT>, lds_v);
+                k_rows_inv<T, NT, MT><<<blocks_of((long long)P * g.nu, NT), NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+            });
+            HIPCHK(hipGetLastError());
+        }
         timer.mark("fft_v");
-        fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-            set_lds(k_cols_inv<T, NT, MT>, lds_u);
-            k_cols_inv<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
-        });
-        HIPCHK(hipGetLastError());
+        if (g.lu1) {
+            long_cols<T, 2>(tmp, P, g.ny, g.lu1, g.lu2, lt.u1, lt.u2, lt.tu1, lt.tu2, lt.tun, +1, proj, g,
+                            tb.screen, st);
+        } else {
+            fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+                set_lds(k_cols_inv<T, NT, MT>, lds_u);
+                k_cols_inv<T, NT, MT><<<blocks_of((long long)(g.ny + C - 1) / C * P, NT), NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+            });
+            HIPCHK(hipGetLastError());
+        }
         timer.mark("fft_u");
         k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
         HIPCHK(hipGetLastError());
@@ -2084,6 +2441,8 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
                      HostKeep& keep, StageTimer& timer) {
     Tables<T> tb;
     make_tables<T>(ws, G, g, st, keep, tb, 1);
+    LongTabs<T> lt;
+    make_long<T>(ws, g, st, keep, lt);
     timer.mark("tables");
     const int P = g.wgrid ? g.nplanes : 1;
     const long long plane = (long long)g.nu * g.nv;
@@ -2094,21 +2453,35 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
     cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
     const int C = col_block(g.nu);
-    fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
-        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-        const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-        set_lds(k_cols_fwd<T, NT, MT>, lds_u);
-        k_cols_fwd<T, NT, MT><<<(g.ny + C - 1) / C * P, NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
-    });
-    HIPCHK(hipGetLastError());
+    if (g.lu1) {
+        k_place_cols<T><<<blocks_of((long long)P * g.nu, 256), 256, 0, st>>>(img, tmp, g, tb.screen);
+        HIPCHK(hipGetLastError());
+        long_cols<T, 0>(tmp, P, g.ny, g.lu1, g.lu2, lt.u1, lt.u2, lt.tu1, lt.tu2, lt.tun, -1, nullptr, g,
+                        tb.screen, st);
+    } else {
+        fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
+            set_lds(k_cols_fwd<T, NT, MT>, lds_u);
+            k_cols_fwd<T, NT, MT><<<blocks_of((long long)(g.ny + C - 1) / C * P, NT), NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+        });
+        HIPCHK(hipGetLastError());
+    }
     timer.mark("fft_u");
-    fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
-        constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-        const std::size_t lds_v = sizeof(cx<T>) * g.nv;
-        set_lds(k_rows_fwd<T, NT, MT>, lds_v);
-        k_rows_fwd<T, NT, MT><<<(long long)P * g.nu, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g.ny);
-    });
-    HIPCHK(hipGetLastError());
+    if (g.lv1) {
+        const long long rows = (long long)P * g.nu;
+        k_place_rows<T><<<blocks_of(rows, 256), 256, 0, st>>>(tmp, grid, g);
+        HIPCHK(hipGetLastError());
+        long_rows<T, 0>(grid, rows, g.lv1, g.lv2, lt.v1, lt.v2, lt.tv1, lt.tv2, lt.tvn, -1, nullptr, g.ny, st);
+    } else {
+        fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+            set_lds(k_rows_fwd<T, NT, MT>, lds_v);
+            k_rows_fwd<T, NT, MT><<<blocks_of((long long)P * g.nu, NT), NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g);
+        });
+        HIPCHK(hipGetLastError());
+    }
     timer.mark("fft_v");
     const int nt = g.ntu * g.ntv;
     std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));

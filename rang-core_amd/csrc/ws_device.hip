@@ -552,9 +552,11 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
         WSCHK(hipMemsetAsync(d_gp, 0, sizeof(cx<T>) * nb * plane, st));
         auto tg = Clock::now();
         const long long s0 = off[layers[l0]], s1 = off[layers[l1 - 1] + 1];
-        if (s1 > s0) {
-            const long long blocks = (s1 - s0 + 3) / 4;
-            k_grid<T><<<blocks, 256, 0, st>>>(d_s, s0, s1, d_slot, g, d_kern, d_gd, d_gp);
+        // a wave per sample: launches of at most 2^24 samples keep each
+        // dispatch within the 2^32 work-items of one grid dimension
+        for (long long c0 = s0; c0 < s1; c0 += (1LL << 24)) {
+            const long long c1 = std::min(s1, c0 + (1LL << 24));
+            k_grid<T><<<(c1 - c0 + 3) / 4, 256, 0, st>>>(d_s, c0, c1, d_slot, g, d_kern, d_gd, d_gp);
             WSCHK(hipGetLastError());
         }
         WSCHK(hipStreamSynchronize(st));

@@ -49,11 +49,14 @@ def lm_of(i, j, n, pix):
     return l, m
 
 
-def test_vis2dirty_pixels_match_the_direct_transform(large):
+@pytest.mark.parametrize("single", [True, False])
+def test_vis2dirty_pixels_match_the_direct_transform(large, single):
+    # the double-precision variant (grid 14000^2) takes the two-pass transforms
     from rang import device as D
 
     c = large
-    d, _, info = D.vis2dirty(opts(c), c["uvw"], c["freq"], c["vis"], c["wgt"], single=True)
+    wgt = c["wgt"] if single else c["wgt"].to(torch.float64)
+    d, _, info = D.vis2dirty(opts(c), c["uvw"], c["freq"], c["vis"], wgt, single=single)
     assert int(info[5]) == c["vis"].numel()
     n, pix = c["npix"], c["pixsize"]
     g = torch.Generator(device="cuda:0").manual_seed(5)
@@ -80,11 +83,13 @@ def test_vis2dirty_pixels_match_the_direct_transform(large):
     got = d[i, j].to(torch.float64)
     peak = float(ref.abs().max())
     err = float((got - ref).abs().max()) / peak
-    print(f"[large] vis2dirty 256 pixels: max error / peak {err:.2e}", flush=True)
+    print(f"[large] vis2dirty ({'f32' if single else 'f64'}, grid {int(info[2])}) 256 pixels: "
+          f"max error / peak {err:.2e}", flush=True)
     assert err < 10 * EPS
 
 
-def test_dirty2vis_of_point_sources_matches_the_direct_transform(large):
+@pytest.mark.parametrize("single", [True, False])
+def test_dirty2vis_of_point_sources_matches_the_direct_transform(large, single):
     from rang import device as D
 
     c = large
@@ -93,8 +98,9 @@ def test_dirty2vis_of_point_sources_matches_the_direct_transform(large):
     pi = rng.integers(n // 4, 3 * n // 4, 20)
     pj = rng.integers(n // 4, 3 * n // 4, 20)
     flux = rng.uniform(0.1, 1.0, 20)
-    img = torch.zeros((n, n), dtype=torch.float32, device="cuda:0")
-    img[torch.from_numpy(pi).cuda(), torch.from_numpy(pj).cuda()] = torch.from_numpy(flux.astype(np.float32)).cuda()
+    real = torch.float32 if single else torch.float64
+    img = torch.zeros((n, n), dtype=real, device="cuda:0")
+    img[torch.from_numpy(pi).cuda(), torch.from_numpy(pj).cuda()] = torch.from_numpy(flux).cuda().to(real)
     vis, _ = D.dirty2vis(opts(c), c["uvw"], c["freq"], img)
     g = torch.Generator(device="cuda:0").manual_seed(9)
     idx = torch.randint(0, vis.numel(), (100_000,), device="cuda:0", generator=g)
@@ -110,7 +116,8 @@ def test_dirty2vis_of_point_sources_matches_the_direct_transform(large):
     ref = torch.complex((f * torch.cos(ph)).sum(1), (f * torch.sin(ph)).sum(1))
     got = vis.reshape(-1)[idx].to(torch.complex128)
     err = float((got - ref).abs().max()) / float(ref.abs().max())
-    print(f"[large] dirty2vis 1e5 visibilities: max error / peak {err:.2e}", flush=True)
+    print(f"[large] dirty2vis ({'f32' if single else 'f64'}) 1e5 visibilities: max error / peak {err:.2e}",
+          flush=True)
     assert err < 10 * EPS
 
 
