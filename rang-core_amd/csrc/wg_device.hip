@@ -198,6 +198,11 @@ struct Geo {
     int Tu, Tv, ntu, ntv;
     int tnx, tny;
     int divide_by_n;
+    // Image-side arrays column-major (index j nx + i): the w-screen and
+    // correction tables, the per-plane projections and the corrected image.
+    // Set with one-pass u transforms, whose workgroups each take whole
+    // image columns.
+    int cm;
     // Transforms longer than one LDS along u or v: n = n1 n2 (0: in LDS).
     // Their forward results sit at (k mod n1) n2 + k / n1.
     int lu1, lu2, lv1, lv2;
@@ -1118,7 +1123,14 @@ __global__ __launch_bounds__(256) void k_tables(Geo g, double* screen, double* c
     const long long total = (long long)g.tnx * g.tny;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int i = int(t / g.tny), j = int(t - (long long)i * g.tny);
+    int i, j;
+    if (g.cm) {
+        j = int(t / g.tnx);
+        i = int(t - (long long)j * g.tnx);
+    } else {
+        i = int(t / g.tny);
+        j = int(t - (long long)i * g.tny);
+    }
     const double x0 = g.lshift - 0.5 * double(g.nx) * g.pixsize_x;
     const double y0 = g.mshift - 0.5 * double(g.ny) * g.pixsize_y;
     const double x = x0 + double(i) * g.pixsize_x, y = y0 + double(j) * g.pixsize_y;
@@ -1139,6 +1151,10 @@ __global__ __launch_bounds__(256) void k_tables(Geo g, double* screen, double* c
 }
 
 __device__ __forceinline__ int table_index(const Geo& g, int i, int j) {
+    if (g.cm) {
+        if (g.lmshift) return j * g.tnx + i;
+        return min(j, g.ny - j) * g.tnx + min(i, g.nx - i);
+    }
     if (g.lmshift) return i * g.tny + j;
     return min(i, g.nx - i) * g.tny + min(j, g.ny - j);
 }
@@ -1180,10 +1196,26 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
     const int nv = d.n;
     const long long row = blockIdx.x;
     const cx<T>* src = grid + row * nv;
-    for (int e = threadIdx.x; e < nv; e += NT) buf[e] = src[e];
+    // every load of the row in flight before the first LDS write
+    constexpr int PER = (MAXTOT + NT - 1) / NT;
+    cx<T> r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < nv) r[k] = src[e];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < nv) buf[e] = r[k];
+    }
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, +1);
-    for (int j = threadIdx.x; j < ny; j += NT) tmp[row * ny + j] = buf[(nv - ny / 2 + j) % nv];
+    const int sh = nv - ny / 2;
+    for (int j = threadIdx.x; j < ny; j += NT) {
+        const int e = sh + j;
+        tmp[row * ny + j] = buf[e >= nv ? e - nv : e];
+    }
 }
 
 // v2d: inverse transform along u of C image columns of one plane, then the
@@ -1202,17 +1234,30 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
     const int p = lin / ncb;
     const int j0 = (lin - p * ncb) * C, nc = min(C, ny - j0);
     const cx<T>* src = tmp + (long long)p * nu * ny;
+    const unsigned mnc = nc > 1 ? unsigned(((1ull << 32) + nc - 1) / nc) : 0u;
     for (int e = threadIdx.x; e < nu * nc; e += NT) {
-        const int u = e / nc, c = e - u * nc;
+        const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
         buf[c * nu + u] = src[(long long)u * ny + j0 + c];
     }
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, +1);
     const double w0 = g.wmin + double(p) * g.dw;
     T* dst = proj + (long long)p * nx * ny;
+    const int sh = nu - nx / 2;
+    const unsigned mnx = unsigned(((1ull << 32) + nx - 1) / nx);
     for (int e = threadIdx.x; e < nx * nc; e += NT) {
-        const int i = e / nc, c = e - i * nc, j = j0 + c;
-        const cx<T> gg = buf[c * nu + (nu - nx / 2 + i) % nu];
+        // column-major: i fastest, so the block writes whole columns
+        int i, c;
+        if (g.cm) {
+            c = int(udiv(unsigned(e), unsigned(nx), mnx));
+            i = e - c * nx;
+        } else {
+            i = int(udiv(unsigned(e), unsigned(nc), mnc));
+            c = e - i * nc;
+        }
+        const int j = j0 + c;
+        const int ui = sh + i;
+        const cx<T> gg = buf[c * nu + (ui >= nu ? ui - nu : ui)];
         T term;
         if (g.wgrid) {
             const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], true);
@@ -1220,7 +1265,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
         } else {
             term = gg.re;
         }
-        dst[(long long)i * ny + j] = term;
+        dst[g.cm ? (long long)j * nx + i : (long long)i * ny + j] = term;
     }
 }
 
@@ -1245,6 +1290,41 @@ __global__ __launch_bounds__(256) void k_sum(const T* __restrict__ proj, T* __re
     }
 }
 
+// k_sum for column-major projections: a 32 x 32 tile of pixels per
+// workgroup, the planes summed along i and the tile written back along j.
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_cm(const T* __restrict__ proj, T* __restrict__ out, Geo g,
+                                                int P, const double* __restrict__ corr,
+                                                const double* __restrict__ cfu,
+                                                const double* __restrict__ cfv) {
+    __shared__ T tile[32][33];
+    const long long npix = (long long)g.nx * g.ny;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int i = i0 + tx;
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int j = j0 + jj;
+        if (i < g.nx && j < g.ny) {
+            const long long e = (long long)j * g.nx + i;
+            T acc = proj[e];
+            for (int p = 1; p < P; ++p) acc = acc + proj[(long long)p * npix + e];
+            T v;
+            if (g.wgrid) {
+                v = acc * T(corr[table_index(g, i, j)]);
+            } else {
+                const int iu = g.nx / 2 - min(i, g.nx - i), iv = g.ny / 2 - min(j, g.ny - j);
+                v = T(double(acc) * cfu[iu] * cfv[iv]);
+            }
+            tile[jj][tx] = v;
+        }
+    }
+    __syncthreads();
+    for (int ii = ty; ii < 32; ii += 8) {
+        const int i2 = i0 + ii, j2 = j0 + tx;
+        if (i2 < g.nx && j2 < g.ny) out[(long long)i2 * g.ny + j2] = tile[tx][ii];
+    }
+}
+
 // d2v: the corrected copy of the image (wgridder.rs:3064-3091).
 template <class T>
 __global__ __launch_bounds__(256) void k_correct(const T* __restrict__ dirty, T* __restrict__ img,
@@ -1263,6 +1343,37 @@ __global__ __launch_bounds__(256) void k_correct(const T* __restrict__ dirty, T*
     }
 }
 
+// k_correct writing the column-major copy: a 32 x 32 tile per workgroup,
+// read along j and written along i.
+template <class T>
+__global__ __launch_bounds__(256) void k_correct_cm(const T* __restrict__ dirty, T* __restrict__ img,
+                                                    Geo g, const double* __restrict__ corr,
+                                                    const double* __restrict__ cfu,
+                                                    const double* __restrict__ cfv) {
+    __shared__ T tile[32][33];
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int ii = ty; ii < 32; ii += 8) {
+        const int i = i0 + ii, j = j0 + tx;
+        if (i < g.nx && j < g.ny) tile[ii][tx] = dirty[(long long)i * g.ny + j];
+    }
+    __syncthreads();
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int i = i0 + tx, j = j0 + jj;
+        if (i < g.nx && j < g.ny) {
+            const T d = tile[tx][jj];
+            T v;
+            if (g.wgrid) {
+                v = d * T(corr[table_index(g, i, j)]);
+            } else {
+                const int iu = g.nx / 2 - min(i, g.nx - i), iv = g.ny / 2 - min(j, g.ny - j);
+                v = d * T(cfu[iu] * cfv[iv]);
+            }
+            img[(long long)j * g.nx + i] = v;
+        }
+    }
+}
+
 // d2v: C image columns times the plane's w-screen placed on the grid rows,
 // forward transform along u (wgridder.rs:2931-2967).
 template <class T, int NT, int MAXTOT>
@@ -1277,26 +1388,59 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     const int p = lin / ncb;
     const int j0 = (lin - p * ncb) * C, nc = min(C, ny - j0);
+    const unsigned mnc = nc > 1 ? unsigned(((1ull << 32) + nc - 1) / nc) : 0u;
+    // the image values of the block in flight while the buffer is cleared
+    constexpr int PER = (MAXTOT + NT - 1) / NT;
+    const unsigned mnx = unsigned(((1ull << 32) + nx - 1) / nx);
+    T r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < nx * nc) {
+            if (g.cm) {
+                // column-major image: the block reads whole columns
+                const int c = int(udiv(unsigned(e), unsigned(nx), mnx)), i = e - c * nx;
+                r[k] = img[(long long)(j0 + c) * nx + i];
+            } else {
+                const int i = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - i * nc;
+                r[k] = img[(long long)i * ny + j0 + c];
+            }
+        }
+    }
     for (int e = threadIdx.x; e < nu * nc; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
     const double w0 = g.wmin + double(p) * g.dw;
-    for (int e = threadIdx.x; e < nx * nc; e += NT) {
-        const int i = e / nc, c = e - i * nc, j = j0 + c;
-        const T v = img[(long long)i * ny + j];
-        cx<T> val;
-        if (g.wgrid) {
-            const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], false);
-            val = {v * ph.re, v * ph.im};
-        } else {
-            val = {v, T(0)};
+    const int sh = nu - nx / 2;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < nx * nc) {
+            int i, c;
+            if (g.cm) {
+                c = int(udiv(unsigned(e), unsigned(nx), mnx));
+                i = e - c * nx;
+            } else {
+                i = int(udiv(unsigned(e), unsigned(nc), mnc));
+                c = e - i * nc;
+            }
+            const int j = j0 + c;
+            const T v = r[k];
+            cx<T> val;
+            if (g.wgrid) {
+                const cx<T> ph = screen_at<T>(w0, screen[table_index(g, i, j)], false);
+                val = {v * ph.re, v * ph.im};
+            } else {
+                val = {v, T(0)};
+            }
+            const int ui = sh + i;
+            buf[c * nu + (ui >= nu ? ui - nu : ui)] = val;
         }
-        buf[c * nu + (nu - nx / 2 + i) % nu] = val;
     }
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, -1);
     cx<T>* dst = tmp + (long long)p * nu * ny;
     for (int e = threadIdx.x; e < nu * nc; e += NT) {
-        const int u = e / nc, c = e - u * nc;
+        const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
         dst[(long long)u * ny + j0 + c] = buf[c * nu + u];
     }
 }
@@ -1313,9 +1457,25 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
     const long long row = blockIdx.x;
     const long long p = row / g.nu;
     const long long src = p * g.nu + upos(g, int(row - p * g.nu));
+    // the image row in flight while the buffer is cleared
+    constexpr int PER = (MAXTOT + NT - 1) / NT;
+    cx<T> r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int j = threadIdx.x + k * NT;
+        if (j < ny) r[k] = tmp[src * ny + j];
+    }
     for (int e = threadIdx.x; e < nv; e += NT) buf[e] = cx<T>{T(0), T(0)};
     __syncthreads();
-    for (int j = threadIdx.x; j < ny; j += NT) buf[(nv - ny / 2 + j) % nv] = tmp[src * ny + j];
+    const int sh = nv - ny / 2;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int j = threadIdx.x + k * NT;
+        if (j < ny) {
+            const int e = sh + j;
+            buf[e >= nv ? e - nv : e] = r[k];
+        }
+    }
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, -1);
     cx<T>* dst = grid + row * nv;
@@ -1998,6 +2158,7 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
         err = "no two-pass split of the grid size for the transforms";
         return false;
     }
+    g.cm = g.lu1 == 0 ? 1 : 0;
     return true;
 }
 
@@ -2376,7 +2537,11 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
             HIPCHK(hipGetLastError());
         }
         timer.mark("fft_u");
-        k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
+        if (g.cm)
+            k_sum_cm<T><<<dim3((g.nx + 31) / 32, (g.ny + 31) / 32), 256, 0, st>>>(proj, outs[oi], g, P, tb.corr,
+                                                                                 tb.cfu, tb.cfv);
+        else
+            k_sum<T><<<(npix + 255) / 256, 256, 0, st>>>(proj, outs[oi], g, P, tb.corr, tb.cfu, tb.cfv);
         HIPCHK(hipGetLastError());
         timer.mark("sum");
     }
@@ -2448,7 +2613,11 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     const long long plane = (long long)g.nu * g.nv;
     const long long npix = (long long)g.nx * g.ny;
     T* img = static_cast<T*>(ws.get(kSlotImg, sizeof(T) * npix));
-    k_correct<T><<<(npix + 255) / 256, 256, 0, st>>>(dirty, img, g, tb.corr, tb.cfu, tb.cfv);
+    if (g.cm)
+        k_correct_cm<T><<<dim3((g.nx + 31) / 32, (g.ny + 31) / 32), 256, 0, st>>>(dirty, img, g, tb.corr,
+                                                                                  tb.cfu, tb.cfv);
+    else
+        k_correct<T><<<(npix + 255) / 256, 256, 0, st>>>(dirty, img, g, tb.corr, tb.cfu, tb.cfv);
     HIPCHK(hipGetLastError());
     cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
     cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
