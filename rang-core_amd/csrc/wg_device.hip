@@ -1688,8 +1688,15 @@ __global__ __launch_bounds__(256) void k_degrid_items(Geo g, const std::uint32_t
 // are first copied to LDS (P (Tu+S-1) (Tv+S-1) values); otherwise the taps
 // read the planes in global memory. Each thread takes a contiguous part of
 // the chunk.
+// Threads per degridding workgroup: staged single precision takes a whole
+// CU (its taps fit 128 registers); the rest run 256-thread workgroups.
+template <class T, bool STAGE>
+constexpr int degrid_threads() {
+    return STAGE && sizeof(T) == 4 ? 1024 : 256;
+}
+
 template <class T, int S, bool STAGE>
-__global__ __launch_bounds__(256) void k_degrid(Geo g, const Rec<T>* __restrict__ recs,
+__global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, const Rec<T>* __restrict__ recs,
                                                 const std::uint32_t* __restrict__ off,
                                                 const T* __restrict__ coeff,
                                                 const cx<T>* __restrict__ grid, std::uint32_t rc,
@@ -1826,7 +1833,7 @@ void launch_degrid(int S, int nblocks, std::size_t lds, hipStream_t st, const Ge
             if (STAGE)                                                                         \
                 HIPCHK(hipFuncSetAttribute((const void*)k_degrid<T, SS, STAGE>,                \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))); \
-            k_degrid<T, SS, STAGE><<<nblocks, 256, lds, st>>>(g, recs, off, coeff, grid, rc,   \
+            k_degrid<T, SS, STAGE><<<nblocks, degrid_threads<T, STAGE>(), lds, st>>>(g, recs, off, coeff, grid, rc, \
                                                               item_off, order, vis);           \
         } else {                                                                               \
             throw DevError{"kernel support too large for this precision"};                     \
@@ -2062,7 +2069,20 @@ constexpr std::size_t grid_lds() {
 // 464M-visibility workload (4096: same, 16384: 1.35x, 65536: 2.6x the time).
 constexpr std::uint32_t kGridChunk = 1024;
 constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
-constexpr std::size_t kDegridLds = 64 * 1024;   // staged planes per workgroup
+// Staged degridding holds a tile and its halo on every plane in LDS: in
+// single precision one 1024-thread workgroup per CU with up to 150 KiB and
+// 4096 visibilities per item, in double precision 256-thread workgroups with
+// up to 64 KiB. Measured on the 464M-visibility workload (26 planes, 16 x 16
+// tiles): 1024 visibilities per item 86.8 ms, 2048 76.3, 4096 73.9, 8192
+// 73.9, 16384 75.6; unstaged (64 KiB limit) 174.7.
+template <class T>
+constexpr std::uint32_t degrid_chunk_staged() {
+    return sizeof(T) == 4 ? 4096 : kDegridChunk;
+}
+template <class T>
+constexpr std::size_t degrid_lds_max() {
+    return sizeof(T) == 4 ? 150 * 1024 : 64 * 1024;
+}
 
 // Longest transform done in one LDS pass (fft_maxtot); longer ones take two
 // passes over memory. RANG_FFT_LDS_MAX lowers the limit (a tuning and test
@@ -2095,7 +2115,7 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
-// every plane fits kDegridLds.
+// every plane fits the staged LDS limit.
 template <class T>
 bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err) {
     g.nx = int(G.nx);
@@ -2131,8 +2151,8 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
     int target = 32;
     if (degrid) {
         // largest side whose staged planes fit, within 48
-        const double side = std::sqrt(double(kDegridLds) / (double(P) * sizeof(cx<T>))) - (g.supp - 1);
-        hi = std::min(48, std::max(int(side), 16));
+        const double side = std::sqrt(double(degrid_lds_max<T>()) / (double(P) * sizeof(cx<T>))) - (g.supp - 1);
+        hi = std::min(48, std::max(int(side), sizeof(T) == 4 ? 8 : 16));
         target = hi;
     }
     g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64), target);
@@ -2655,14 +2675,16 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     const int nt = g.ntu * g.ntv;
     std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
     std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
-    k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, kDegridChunk, items);
+    const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
+    const bool staged = dl <= degrid_lds_max<T>();
+    const std::uint32_t rc = staged ? degrid_chunk_staged<T>() : kDegridChunk;
+    k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, rc, items);
     HIPCHK(hipGetLastError());
     exclusive_scan(ws, items, item_off, nt + 1, st);
     const long long ub = read_u32(ws, item_off + nt, st);
     if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
-    const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
-    if (dl <= kDegridLds)
-        launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, kDegridChunk,
+    if (staged)
+        launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, rc,
                                item_off, tb.order, cvis);
     else
         launch_degrid<T, false>(g.supp, int(ub), 0, st, g, recs, off, tb.coeff, grid, kDegridChunk,
