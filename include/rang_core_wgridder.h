@@ -234,6 +234,70 @@ int32_t rang_amd_wgridder_prepared_cg_f64(RangPrepared *p, const double *rhs, do
 int32_t rang_amd_wgridder_prepared_cg_f32(RangPrepared *p, const float *rhs, float *x,
                                           int32_t iterations, double tolerance, int32_t *used,
                                           double *rel, void *stream);
+/* prox_solver::solve (core/src/prox_solver.rs:283-445), FISTA or forward-
+ * backward for 1/2 |y - Phi m|^2_W + R(m) on the prepared plan's normal
+ * operator (weights as prepared, normally normalised to sum to one), with
+ * the L1 prior R(x) = sum lambda factor_i |x_i| (L1Prior, :55-132; factors
+ * NULL: all one) and x >= 0 unless `signed_`. `dirty` = Phi^H W y and q0 =
+ * 1/2 |y|^2_W (Problem, :167-172); x holds the start on entry and the result
+ * on return, gx (may be NULL) G x. Settings as the reference's (:175-208;
+ * a NaN tolerance is None; objective_every does not apply to this prior).
+ * One Gram application per iteration; records[k] (up to nrecords) is the
+ * reference's Record of iterate k. Image vectors nx x ny on the device. */
+typedef struct RangFistaSettings {
+    double step;
+    uint64_t max_iterations;
+    int32_t momentum;
+    int32_t restart;
+    double gap_tolerance;
+    double kkt_tolerance;
+    double step_tolerance;
+} RangFistaSettings;
+typedef struct RangFistaRecord {
+    uint64_t iteration;
+    double objective;
+    double misfit;
+    double penalty;
+    double gap;
+    double kkt;
+    uint64_t nonzero;
+    double flux;
+    double curvature;
+    double relative_step;
+    int32_t restarted;
+    int32_t reserved;
+} RangFistaRecord;
+typedef struct RangFistaOutcome {
+    uint64_t iterations;
+    int32_t converged; /* stop: 1 "converged", 0 "iterations" */
+    int32_t reserved;
+    uint64_t restarts;
+    uint64_t objective_rises;
+    double largest_rise;
+    uint64_t negative_curvatures;
+    double min_curvature;
+    double max_curvature;
+    RangFistaRecord last;
+} RangFistaOutcome;
+int32_t rang_amd_wgridder_prepared_fista_f64(RangPrepared *p, const double *dirty, double q0,
+                                             double lambda, const double *factors, int32_t signed_,
+                                             const RangFistaSettings *settings, double *x,
+                                             double *gx, RangFistaRecord *records,
+                                             size_t nrecords, RangFistaOutcome *out, void *stream);
+int32_t rang_amd_wgridder_prepared_fista_f32(RangPrepared *p, const float *dirty, double q0,
+                                             double lambda, const float *factors, int32_t signed_,
+                                             const RangFistaSettings *settings, float *x,
+                                             float *gx, RangFistaRecord *records, size_t nrecords,
+                                             RangFistaOutcome *out, void *stream);
+/* prox_solver::power_iteration (:886-908): the largest eigenvalue of the normal
+ * operator from the reference's seeded start (xorshift64, :871-882), stopping
+ * at a relative change <= tolerance; *used applications. */
+int32_t rang_amd_wgridder_prepared_power_f64(RangPrepared *p, uint64_t seed, double tolerance,
+                                             uint64_t max_iterations, double *value,
+                                             uint64_t *used, void *stream);
+int32_t rang_amd_wgridder_prepared_power_f32(RangPrepared *p, uint64_t seed, double tolerance,
+                                             uint64_t max_iterations, double *value,
+                                             uint64_t *used, void *stream);
 /* Prepared::active (:3257-3262), memory (:3386-3418, total device bytes),
  * release_grids (:3420-3429), and the plan's release. */
 size_t rang_amd_wgridder_prepared_active(const RangPrepared *p);

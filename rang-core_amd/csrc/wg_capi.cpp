@@ -608,6 +608,29 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions* opts, size_t 
             *used = u;                                                                              \
             return st;                                                                              \
         });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_fista_##SUF(RangPrepared* h, const T* dirty, double q0,     \
+                                                   double lambda, const T* factors, int32_t sgn,   \
+                                                   const RangFistaSettings* settings, T* x, T* gx, \
+                                                   RangFistaRecord* records, size_t nrecords,      \
+                                                   RangFistaOutcome* out, void* stream) {          \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!dirty || !settings || !x || !out) return err = "null array", 1;                   \
+            if (!(settings->step > 0.0)) return err = "the step must be positive", 1;              \
+            if (!(lambda >= 0.0)) return err = "the regularisation weight must be non-negative", 1; \
+            return prepared_fista<T>(p, dirty, q0, lambda, factors, sgn != 0, *settings, x, gx,    \
+                                     records, nrecords, out, static_cast<hipStream_t>(stream), err); \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_power_##SUF(RangPrepared* h, uint64_t seed, double tolerance, \
+                                                   uint64_t max_iterations, double* value,          \
+                                                   uint64_t* used, void* stream) {                  \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!value || !used) return err = "null array", 1;                                     \
+            if (max_iterations == 0) return err = "at least one iteration", 1;                     \
+            return prepared_power<T>(p, seed, tolerance, max_iterations, value, used,              \
+                                     static_cast<hipStream_t>(stream), err);                       \
+        });                                                                                         \
     }
 RANG_PREPARED_ABI(f64, double)
 RANG_PREPARED_ABI(f32, float)

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <string>
 
+#include "../../include/rang_core_wgridder.h"
 #include "wg_plan.h"
 
 namespace rang {
@@ -74,6 +75,13 @@ int prepared_gram(PreparedBase* p, const T* x, T* out, hipStream_t stream, std::
 template <class T>
 int prepared_cg(PreparedBase* p, const T* rhs, T* x, int iterations, double tolerance, int* used,
                 double* rel, hipStream_t stream, std::string& err);
+template <class T>
+int prepared_fista(PreparedBase* p, const T* dirty, double q0, double lam, const T* factors, int sgn,
+                   const RangFistaSettings& settings, T* x, T* gx, RangFistaRecord* records,
+                   std::size_t nrecords, RangFistaOutcome* out, hipStream_t stream, std::string& err);
+template <class T>
+int prepared_power(PreparedBase* p, std::uint64_t seed, double tolerance, std::uint64_t max_iterations,
+                   double* value, std::uint64_t* used, hipStream_t stream, std::string& err);
 
 // Per-kernel timing of the last call on this thread (hipEvents on the call's
 // stream; enabled by set_stage_timing). Names and milliseconds.

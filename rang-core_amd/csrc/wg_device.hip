@@ -97,6 +97,7 @@ enum Slot : int {
     kSlotRed,
     kSlotSolve,
     kSlotLong,  // twiddles of the transforms longer than one LDS
+    kSlotFista,  // the image vectors of the FISTA solve
     kSlotCount
 };
 
@@ -2468,6 +2469,113 @@ __global__ __launch_bounds__(256) void k_cg_dir(T* p, const T* r, double beta, l
         p[i] = T(double(r[i]) + beta * double(p[i]));
 }
 
+// FISTA (core/src/prox_solver.rs:283-445). Per iteration the sums
+// <d, x>, <x, G x>, the penalty, |x - x_prev|^2, |x|^2, the flux, the
+// nonzero count and the curvature's numerator and denominator, then the
+// largest optimality violation and the smallest dual scale bound.
+constexpr int kFistaSums = 9;
+constexpr int kFistaSlots = kFistaSums + 2;
+
+__device__ __forceinline__ void block_reduce_fista(double (&v)[kFistaSlots], double* part) {
+    __shared__ double s[256];
+    for (int q = 0; q < kFistaSlots; ++q) {
+        s[threadIdx.x] = v[q];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (int(threadIdx.x) < h) {
+                const double a = s[threadIdx.x], b = s[threadIdx.x + h];
+                s[threadIdx.x] = q < kFistaSums ? a + b : q == kFistaSums ? fmax(a, b) : fmin(a, b);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(long long)blockIdx.x * kFistaSlots + q] = s[0];
+        __syncthreads();
+    }
+}
+
+// G x by linearity from G y (prox_solver.rs:332-341), the curvature terms
+// (:342-353) and the certificate of x (:236-281) with the per-pixel
+// thresholds lam * factor.
+template <class T>
+__global__ __launch_bounds__(256) void k_fista_eval(const T* y, const T* y_prev, const T* gy,
+                                                    const T* gy_prev, const T* gx_prev, T* gx, const T* x,
+                                                    const T* x_prev, const T* d, const T* factors,
+                                                    double lam, int sgn, double beta, long long n,
+                                                    double* part) {
+    double v[kFistaSlots];
+    for (int q = 0; q < kFistaSums; ++q) v[q] = 0.0;
+    v[kFistaSums] = 0.0;
+    v[kFistaSums + 1] = INFINITY;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double gyi = double(gy[i]);
+        const double gxi = beta == 0.0 ? gyi : (gyi + beta * double(gx_prev[i])) / (1.0 + beta);
+        gx[i] = T(gxi);
+        const double dy = double(y[i]) - double(y_prev[i]);
+        v[7] += dy * (gyi - double(gy_prev[i]));
+        v[8] += dy * dy;
+        const double xi = double(x[i]), di = double(d[i]);
+        v[0] += di * xi;
+        v[1] += xi * gxi;
+        const double r = di - gxi;
+        const double lami = factors ? lam * double(factors[i]) : lam;
+        double viol;
+        if (xi > 0.0)
+            viol = fabs(r - lami);
+        else if (xi < 0.0)
+            viol = fabs(r + lami);
+        else if (sgn)
+            viol = fmax(fabs(r) - lami, 0.0);
+        else
+            viol = fmax(r - lami, 0.0);
+        const double excess = sgn ? fabs(r) : r;
+        const double bound = excess > 0.0 ? lami / excess : INFINITY;
+        v[2] += lami * fabs(xi);
+        v[kFistaSums] = fmax(v[kFistaSums], viol);
+        v[kFistaSums + 1] = fmin(v[kFistaSums + 1], bound);
+        const double ch = xi - double(x_prev[i]);
+        v[3] += ch * ch;
+        v[4] += xi * xi;
+        v[5] += xi;
+        v[6] += xi != 0.0 ? 1.0 : 0.0;
+    }
+    block_reduce_fista(v, part);
+}
+
+// The forward-backward step from y and the prox of the L1 prior
+// (prox_solver.rs:407-412, 80-101), and the partials of the restart test
+// <y - x_next, x_next - x> (:416-419).
+template <class T>
+__global__ __launch_bounds__(256) void k_fista_step(const T* y, const T* d, const T* gy, const T* x, T* x_next,
+                                                    const T* factors, double lam, int sgn, double step,
+                                                    long long n, double* part) {
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double yi = double(y[i]);
+        const double z = yi + step * (double(d[i]) - double(gy[i]));
+        const double tau = factors ? step * lam * double(factors[i]) : step * lam;
+        // f64::signum: +-1 by the sign bit
+        const double xn = sgn ? copysign(1.0, z) * fmax(fabs(z) - tau, 0.0) : fmax(z - tau, 0.0);
+        x_next[i] = T(xn);
+        q += (yi - xn) * (xn - double(x[i]));
+    }
+    block_sum(q, part);
+}
+
+// y = x + beta (x - x_prev) (prox_solver.rs:438-440).
+template <class T>
+__global__ __launch_bounds__(256) void k_fista_extrap(T* y, const T* x, const T* x_prev, double beta,
+                                                      long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        y[i] = T(double(x[i]) + beta * (double(x[i]) - double(x_prev[i])));
+}
+
+// v *= s.
+template <class T>
+__global__ __launch_bounds__(256) void k_scale(T* v, double s, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        v[i] = T(double(v[i]) * s);
+}
+
 }  // namespace
 
 void set_stage_timing(bool on) { g_timing = on; }
@@ -3092,6 +3200,217 @@ int prepared_cg(PreparedBase* base, const T* rhs, T* x, int iterations, double t
     }
 }
 
+namespace {
+
+// The reference's seeded start vectors (core/src/prox_solver.rs:871-882):
+// xorshift64 from max(seed, 1), each value (state >> 11) / 2^53 - 1/2.
+std::vector<double> seeded_vector(std::size_t n, std::uint64_t seed) {
+    std::vector<double> v(n);
+    std::uint64_t state = std::max<std::uint64_t>(seed, 1);
+    for (std::size_t i = 0; i < n; ++i) {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        v[i] = double(state >> 11) / double(1ull << 53) - 0.5;
+    }
+    return v;
+}
+
+// Sums, the largest violation and the smallest bound of k_fista_eval's
+// partials, combined on the host in block order.
+template <class T>
+void fista_partials(Workspace& ws, const double* part, int nblk, hipStream_t st, double (&v)[kFistaSlots]) {
+    double* h = static_cast<double*>(ws.host(sizeof(double) * kFistaSlots * nblk));
+    HIPCHK(hipMemcpyAsync(h, part, sizeof(double) * kFistaSlots * nblk, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int q = 0; q < kFistaSums; ++q) v[q] = 0.0;
+    v[kFistaSums] = 0.0;
+    v[kFistaSums + 1] = INFINITY;
+    for (int b = 0; b < nblk; ++b) {
+        const double* r = h + (long long)b * kFistaSlots;
+        for (int q = 0; q < kFistaSums; ++q) v[q] += r[q];
+        v[kFistaSums] = std::fmax(v[kFistaSums], r[kFistaSums]);
+        v[kFistaSums + 1] = std::fmin(v[kFistaSums + 1], r[kFistaSums + 1]);
+    }
+}
+
+}  // namespace
+
+// FISTA or forward-backward on the prepared plan's normal operator
+// (core/src/prox_solver.rs:283-445) with the L1 prior lam * factor_i (x >= 0
+// unless signed): one Gram application per iteration, G x by linearity, and
+// per iterate the record the reference reports (objective, misfit, penalty,
+// duality gap, optimality residual, nonzero count, flux, curvature, relative
+// step, restart). Vectors in T on the device, sums in double.
+template <class T>
+int prepared_fista(PreparedBase* base, const T* dirty, double q0, double lam, const T* factors, int sgn,
+                   const RangFistaSettings& set, T* x_io, T* gx_out, RangFistaRecord* records,
+                   std::size_t nrecords, RangFistaOutcome* outc, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        const long long m = (long long)p.o.nx * p.o.ny;
+        const std::size_t vb = sizeof(T) * std::max<long long>(m, 1);
+        T* work = static_cast<T*>(p.ws.get(kSlotFista, vb * 9));
+        T* x = work;
+        T* x_prev = work + m;
+        T* x_next = work + 2 * m;
+        T* y = work + 3 * m;
+        T* y_prev = work + 4 * m;
+        T* gy = work + 5 * m;
+        T* gy_prev = work + 6 * m;
+        T* gx = work + 7 * m;
+        T* gx_prev = work + 8 * m;
+        HIPCHK(hipMemcpyAsync(x, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(x_prev, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(y, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemsetAsync(y_prev, 0, sizeof(T) * m, st));
+        HIPCHK(hipMemsetAsync(gy_prev, 0, sizeof(T) * m, st));
+        HIPCHK(hipMemsetAsync(gx_prev, 0, sizeof(T) * m, st));
+        const int nblk = red_blocks(m);
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kFistaSlots * kRedBlocks));
+        RangFistaOutcome out{};
+        out.min_curvature = INFINITY;
+        out.max_curvature = -INFINITY;
+        double t = 1.0, beta = 0.0, last_objective = INFINITY;
+        bool restarted = false;
+        const bool have_gap = !std::isnan(set.gap_tolerance), have_kkt = !std::isnan(set.kkt_tolerance);
+        const bool have_step = !std::isnan(set.step_tolerance);
+        for (std::uint64_t k = 0;; ++k) {
+            gram_apply<T>(p, y, gy, st, keep, timer);
+            k_fista_eval<T><<<nblk, 256, 0, st>>>(y, y_prev, gy, gy_prev, gx_prev, gx, x, x_prev, dirty,
+                                                   factors, lam, sgn, beta, m, part);
+            HIPCHK(hipGetLastError());
+            double v[kFistaSlots];
+            fista_partials<T>(p.ws, part, nblk, st, v);
+            double curvature = NAN;
+            if (k > 0 && v[8] > 0.0) curvature = v[7] / v[8];
+            if (std::isfinite(curvature)) {
+                out.min_curvature = std::fmin(out.min_curvature, curvature);
+                out.max_curvature = std::fmax(out.max_curvature, curvature);
+                if (curvature < 0.0) ++out.negative_curvatures;
+            }
+            // certificate (prox_solver.rs:236-281)
+            const double d_x = v[0];
+            const double misfit = q0 - d_x + 0.5 * v[1];
+            const double pen = v[2], kkt = v[kFistaSums], scale_max = v[kFistaSums + 1];
+            const double y_e = 2.0 * q0 - d_x, e_e = 2.0 * misfit;
+            const double sc = std::min(std::max(y_e / std::max(e_e, 1e-300), 0.0), scale_max);
+            const double dual = sc * y_e - 0.5 * sc * sc * e_e;
+            const double gap = misfit + pen - dual;
+            RangFistaRecord rec{};
+            rec.iteration = k;
+            rec.objective = misfit + pen;
+            rec.misfit = misfit;
+            rec.penalty = pen;
+            rec.gap = gap;
+            rec.kkt = kkt;
+            rec.nonzero = std::uint64_t(v[6]);
+            rec.flux = v[5];
+            rec.curvature = curvature;
+            rec.relative_step = std::sqrt(v[3]) / std::max(std::sqrt(v[4]), 1e-300);
+            rec.restarted = restarted ? 1 : 0;
+            if (records && k < nrecords) records[k] = rec;
+            if (std::isfinite(rec.objective)) {
+                if (rec.objective > last_objective) {
+                    ++out.objective_rises;
+                    out.largest_rise = std::fmax(out.largest_rise, rec.objective - last_objective);
+                }
+                last_objective = rec.objective;
+            }
+            out.last = rec;
+            out.iterations = k;
+            const bool converged = (have_gap && gap <= set.gap_tolerance) || (have_kkt && kkt <= set.kkt_tolerance) ||
+                                   (k > 0 && have_step && rec.relative_step <= set.step_tolerance);
+            if (converged || k == set.max_iterations) {
+                out.converged = converged ? 1 : 0;
+                HIPCHK(hipMemcpyAsync(x_io, x, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+                if (gx_out) HIPCHK(hipMemcpyAsync(gx_out, gx, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+                HIPCHK(hipStreamSynchronize(st));
+                break;
+            }
+            k_fista_step<T><<<nblk, 256, 0, st>>>(y, dirty, gy, x, x_next, factors, lam, sgn, set.step, m, part);
+            HIPCHK(hipGetLastError());
+            const double inner = sum_partials<T>(p.ws, part, nblk, st);
+            double next_beta = 0.0;
+            restarted = false;
+            if (set.momentum) {
+                if (set.restart && inner > 0.0) {
+                    t = 1.0;
+                    restarted = true;
+                    ++out.restarts;
+                } else {
+                    const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+                    next_beta = (t - 1.0) / t_next;
+                    t = t_next;
+                }
+            }
+            std::swap(y, y_prev);
+            std::swap(gy, gy_prev);
+            std::swap(gx, gx_prev);
+            std::swap(x_prev, x);
+            std::swap(x, x_next);
+            k_fista_extrap<T><<<nblk, 256, 0, st>>>(y, x, x_prev, next_beta, m);
+            HIPCHK(hipGetLastError());
+            beta = next_beta;
+        }
+        timer.finish();
+        *outc = out;
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+// The largest eigenvalue of the normal operator by power iteration from the
+// reference's seeded start (core/src/prox_solver.rs:886-908): the estimate
+// and the applications used.
+template <class T>
+int prepared_power(PreparedBase* base, std::uint64_t seed, double tolerance, std::uint64_t max_iterations,
+                   double* value_out, std::uint64_t* used, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        const long long m = (long long)p.o.nx * p.o.ny;
+        T* work = static_cast<T*>(p.ws.get(kSlotSolve, sizeof(T) * 2 * std::max<long long>(m, 1)));
+        T *v = work, *g = work + m;
+        const std::vector<double> s0 = seeded_vector(std::size_t(m), seed);
+        std::vector<T> h(s0.begin(), s0.end());
+        HIPCHK(hipMemcpyAsync(v, h.data(), sizeof(T) * m, hipMemcpyHostToDevice, st));
+        const int nblk = red_blocks(m);
+        const double norm = std::sqrt(dot<T>(p, v, v, m, st));
+        k_scale<T><<<nblk, 256, 0, st>>>(v, 1.0 / norm, m);
+        HIPCHK(hipGetLastError());
+        double value = 0.0;
+        std::uint64_t k = 1;
+        for (; k <= max_iterations; ++k) {
+            gram_apply<T>(p, v, g, st, keep, timer);
+            const double next = std::sqrt(dot<T>(p, g, g, m, st));
+            k_scale<T><<<nblk, 256, 0, st>>>(g, 1.0 / std::max(next, 1e-300), m);
+            HIPCHK(hipGetLastError());
+            std::swap(v, g);
+            if (k > 1 && std::fabs(next - value) <= tolerance * next) {
+                value = next;
+                break;
+            }
+            value = next;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        *value_out = value;
+        *used = std::min(k, max_iterations);
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
 #define RANG_PREPARED_INST(T)                                                                     \
     template PreparedBase* prepared_new<T>(const Options&, const DevInputs&, const double*,         \
                                            hipStream_t, int&, std::string&);                        \
@@ -3102,7 +3421,12 @@ int prepared_cg(PreparedBase* base, const T* rhs, T* x, int iterations, double t
                                       hipStream_t, std::string&);                                   \
     template int prepared_gram<T>(PreparedBase*, const T*, T*, hipStream_t, std::string&);         \
     template int prepared_cg<T>(PreparedBase*, const T*, T*, int, double, int*, double*, hipStream_t, \
-                                std::string&);
+                                std::string&);                                                      \
+    template int prepared_fista<T>(PreparedBase*, const T*, double, double, const T*, int,          \
+                                   const RangFistaSettings&, T*, T*, RangFistaRecord*, std::size_t, \
+                                   RangFistaOutcome*, hipStream_t, std::string&);                   \
+    template int prepared_power<T>(PreparedBase*, std::uint64_t, double, std::uint64_t, double*,    \
+                                   std::uint64_t*, hipStream_t, std::string&);
 RANG_PREPARED_INST(float)
 RANG_PREPARED_INST(double)
 #undef RANG_PREPARED_INST

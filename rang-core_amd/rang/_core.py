@@ -51,6 +51,58 @@ _vp, _sz, _i32, _dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c
 _OPT = ctypes.POINTER(WgridderOptions)
 
 
+class FistaSettings(ctypes.Structure):
+    """RangFistaSettings (include/rang_core_wgridder.h): prox_solver::Settings
+    (core/src/prox_solver.rs:175-208), NaN for a tolerance that is None."""
+
+    _fields_ = [
+        ("step", ctypes.c_double),
+        ("max_iterations", ctypes.c_uint64),
+        ("momentum", ctypes.c_int32),
+        ("restart", ctypes.c_int32),
+        ("gap_tolerance", ctypes.c_double),
+        ("kkt_tolerance", ctypes.c_double),
+        ("step_tolerance", ctypes.c_double),
+    ]
+
+
+class FistaRecord(ctypes.Structure):
+    """RangFistaRecord: prox_solver::Record (core/src/prox_solver.rs:211-231)."""
+
+    _fields_ = [
+        ("iteration", ctypes.c_uint64),
+        ("objective", ctypes.c_double),
+        ("misfit", ctypes.c_double),
+        ("penalty", ctypes.c_double),
+        ("gap", ctypes.c_double),
+        ("kkt", ctypes.c_double),
+        ("nonzero", ctypes.c_uint64),
+        ("flux", ctypes.c_double),
+        ("curvature", ctypes.c_double),
+        ("relative_step", ctypes.c_double),
+        ("restarted", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class FistaOutcome(ctypes.Structure):
+    """RangFistaOutcome: prox_solver::Outcome without x and G x
+    (core/src/prox_solver.rs:234-249)."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_uint64),
+        ("converged", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("restarts", ctypes.c_uint64),
+        ("objective_rises", ctypes.c_uint64),
+        ("largest_rise", ctypes.c_double),
+        ("negative_curvatures", ctypes.c_uint64),
+        ("min_curvature", ctypes.c_double),
+        ("max_curvature", ctypes.c_double),
+        ("last", FistaRecord),
+    ]
+
+
 class GridInputs(ctypes.Structure):
     """`GridInputs` (core/src/ffi_grid.rs:7-29)."""
 
@@ -112,6 +164,12 @@ SIGNATURES = {
     "rang_amd_wgridder_prepared_gram_f32": ([_vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_cg_f64": ([_vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_cg_f32": ([_vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_fista_f64": (
+        [_vp, _vp, _dp, _dp, _vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_fista_f32": (
+        [_vp, _vp, _dp, _dp, _vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_power_f64": ([_vp, ctypes.c_uint64, _dp, ctypes.c_uint64, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_power_f32": ([_vp, ctypes.c_uint64, _dp, ctypes.c_uint64, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_active": ([_vp], _sz),
     "rang_amd_wgridder_prepared_memory": ([_vp], _sz),
     "rang_amd_wgridder_prepared_release_grids": ([_vp], None),

@@ -105,6 +105,27 @@ class Prepared:
         _check(self._fn("gram")(self._h, _ptr(x), _ptr(d), _stream()))
         return d
 
+    def fista(self, dirty, q0, lam, x, *, factors=None, signed=False, settings, gx=None, nrecords=0):
+        """prox_solver::solve (core/src/prox_solver.rs:283-445) with the L1 prior
+        on this plan's normal operator. x (the start) is overwritten with the
+        result; returns (outcome, records[:min(iterations + 1, nrecords)])."""
+        recs = (_core.FistaRecord * max(int(nrecords), 1))()
+        out = _core.FistaOutcome()
+        _check(self._fn("fista")(self._h, _ptr(dirty), float(q0), float(lam), _ptr(factors),
+                                  int(bool(signed)), ctypes.byref(settings), _ptr(x), _ptr(gx), recs,
+                                  int(nrecords), ctypes.byref(out), _stream()))
+        n = min(int(out.iterations) + 1, int(nrecords))
+        return out, [recs[k] for k in range(n)]
+
+    def power_iteration(self, seed, tolerance, max_iterations):
+        """prox_solver::power_iteration (core/src/prox_solver.rs:886-908):
+        (largest eigenvalue of A^H W A, applications used)."""
+        value = ctypes.c_double(0.0)
+        used = ctypes.c_uint64(0)
+        _check(self._fn("power")(self._h, int(seed), float(tolerance), int(max_iterations),
+                                  ctypes.byref(value), ctypes.byref(used), _stream()))
+        return value.value, int(used.value)
+
     def conjugate_gradients(self, rhs, iterations, tolerance, out=None):
         """CG on A^H W A x = rhs from zero (core/src/clean.rs:263-306):
         (x, Gram applications used, |rhs - A^H W A x| / |rhs|)."""

@@ -20,11 +20,21 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 45
+    assert len(names) == 49
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(_core.SIGNATURES)
+
+
+def test_fista_structs_match_the_header():
+    # RangFistaSettings / Record / Outcome as laid out by a C compiler
+    import ctypes
+
+    assert ctypes.sizeof(_core.FistaSettings) == 48
+    assert ctypes.sizeof(_core.FistaRecord) == 88
+    assert ctypes.sizeof(_core.FistaOutcome) == 64 + 88
+    assert _core.FistaOutcome.last.offset == 64
 
 
 def test_options_struct_layout():
