@@ -180,6 +180,34 @@ SIGNATURES = {
     "rang_core_grid_kernel": ([_sz, _sz, _vp], _i32),
     "rang_amd_grid_last_error": ([_vp, _sz], _sz),
     "rang_amd_device_name": ([_vp, _sz], _sz),
+    # the learned preconditioner's kernels (include/rang_core_precond.h)
+    "rang_amd_nn_last_error": ([_vp, _sz], _sz),
+    "rang_amd_nn_conv_fwd": ([_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32,
+                              _i32, _vp], _i32),
+    "rang_amd_nn_conv_wgrad": ([_vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32,
+                                _vp], _i32),
+    "rang_amd_nn_flip": ([_vp, _vp, _i32, _i32, _i32, _vp], _i32),
+    "rang_amd_nn_to_bf16": ([_vp, _vp, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_relu_bwd": ([_vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _i32, ctypes.c_int64, _i32, _vp],
+                             _i32),
+    "rang_amd_nn_pool": ([_vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp], _i32),
+    "rang_amd_nn_pool_bwd": ([_vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp], _i32),
+    "rang_amd_nn_up": ([_vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp], _i32),
+    "rang_amd_nn_up_bwd": ([_vp, _i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp], _i32),
+    "rang_amd_nn_adam": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_pack_f64": ([_vp, _vp, _vp, _i32, _i32, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_pack_f32": ([_vp, _vp, _vp, _i32, _i32, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_unpack_f64": ([_vp, _i32, _i32, _vp, _vp, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_unpack_f32": ([_vp, _i32, _i32, _vp, _vp, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_image_dot_f64": ([_vp, _vp, _vp, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_image_dot_f32": ([_vp, _vp, _vp, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_loss": ([_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i32, ctypes.c_int64, _vp, _vp], _i32),
+    "rang_amd_nn_sky_f64": ([_vp, _i32, _i32, _i32, _i32, _dp, _dp, _i32, ctypes.c_uint64, _vp], _i32),
+    "rang_amd_nn_sky_f32": ([_vp, _i32, _i32, _i32, _i32, _dp, _dp, _i32, ctypes.c_uint64, _vp], _i32),
+    "rang_amd_nn_scale_images": ([_vp, _vp, _vp, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_grad_images": ([_vp, _vp, _vp, _i32, _i32, _i32, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_axpby_f64": ([_vp, _vp, _dp, _dp, ctypes.c_int64, _vp], _i32),
+    "rang_amd_nn_axpby_f32": ([_vp, _vp, _dp, _dp, ctypes.c_int64, _vp], _i32),
     "rang_amd_stage_timing": ([_i32], None),
     "rang_amd_stage_count": ([], _i32),
     "rang_amd_stage_name": ([_i32], ctypes.c_char_p),

@@ -10,7 +10,8 @@ from conftest import ROOT, ref_case
 from oracle import oracle as O
 from rang import _core, core_wgridder
 
-HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h"]
+HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h",
+           ROOT / "include" / "rang_core_precond.h"]
 
 
 def declared():
@@ -20,7 +21,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 49
+    assert len(names) == 73
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
