@@ -173,7 +173,7 @@ def test_unet_gradients_match_autograd():
     xr = nchw(from_bits(bf16_bits(x)), B, H, W, P.CIN)
     out, ws = torch_unet(net, xr)
     ref_loss = sum(float(inv[n]) * 0.5 * ((out[n, 0].reshape(-1) - t[n]) ** 2).sum() for n in range(B))
-    assert abs(float(loss[0]) - float(ref_loss)) <= 2e-2 * float(ref_loss)
+    assert abs(float(loss[0]) - float(ref_loss.detach())) <= 2e-2 * float(ref_loss.detach())
     ref_loss.backward()
     for k, w in ws:
         got = torch_weight(k.g.cpu(), k.ks).reshape(-1).double()
