@@ -15,8 +15,11 @@ The JSON line also carries: the roofline of the step's dominant kernel
 (algorithmic bytes of one launch over its HIP-event duration on the call's
 stream, against 8 TB/s); the oracle (the CPU restatement of the reference
 gridder, OpenMP) timed on this host's cores on a bounded sample of the same
-workload and scaled to it; and the same measurements on the small workload
-(3,225,600 visibilities, 1024 x 1024 at 2 arcsec) under "config_small".
+workload and scaled to it; the same measurements on the small workload
+(3,225,600 visibilities, 1024 x 1024 at 2 arcsec) under "config_small"; and,
+under "gram_solve", the Gram applications plain CG and the learned
+preconditioner need to reach a relative residual of 1e-3 on a reduced
+certified-imaging case (--no-gram skips it).
 """
 
 import argparse
@@ -253,6 +256,55 @@ def cpu_large(info, nvis_total):
     }
 
 
+def gram_solve(dev, tol=1e-3, budget=2000, steps=400):
+    """The Gram-solve half of the metric on a reduced certified-imaging case
+    (2016 baselines x 32 times x 8 channels, 128^2 at 2 arcsec, float64,
+    epsilon 1e-7): Gram applications to a relative residual `tol` for plain
+    CG and for flexible CG with the learned preconditioner (trained on this
+    plan's own residuals; its training applications reported beside), each
+    result's residual recomputed with the exact Gram operator."""
+    import torch
+
+    from rang import precond as P
+    from rang import synth
+    from rang.measurement import WgridMeasurement, conjugate_gradients
+
+    c = synth.imaging_case_device(32, 8, 1.30e9, 1.45e9, 128, 2.0, dev)
+    m = WgridMeasurement(c["uvw"], c["freq"], c["vis"], c["wgt"].double(), 128, c["pixsize"], 1e-7)
+    rhs, _ = m.residual(torch.zeros((128, 128), dtype=torch.float64, device=dev))
+    t0 = time.perf_counter()
+    _, used_cg, rel_cg = conjugate_gradients(m, rhs, budget, tol)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    net = P.UNet(128, 128, batch=8, levels=3, base=32, seed=2)
+    tr = P.Trainer(m, net, seed=11)
+    t0 = time.perf_counter()
+    tr.collect(rhs_count=16, iterations=200)
+    tr.train(steps, lr=2e-3, lr_end=1e-4)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    M = P.Preconditioner(m, net)
+    t0 = time.perf_counter()
+    _, used_p, rel_p = P.preconditioned_cg(m, M, rhs, budget, tol)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t0
+    return {
+        "workload": "2016 baselines x 32 times x 8 channels (516,096 vis), 128x128 at 2 arcsec, "
+                    "eps 1e-7, float64; A^H W A x = A^H W v from zero",
+        "tolerance": tol,
+        "budget": budget,
+        "cg_applies": used_cg,
+        "cg_relative_residual": rel_cg,
+        "cg_seconds": round(t_cg, 3),
+        "learned_applies": used_p,
+        "learned_exact_relative_residual": rel_p,
+        "learned_seconds": round(t_p, 3),
+        "training_gram_applies": tr.gram_applies,
+        "training_seconds": round(t_train, 2),
+        "network": f"U-Net 3 levels x 32 base channels, BF16 on MFMA, {net.parameters()} weights",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +313,7 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the small workload")
+    ap.add_argument("--no-gram", action="store_true", help="skip the Gram-solve comparison")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -325,6 +378,10 @@ def main():
             print(json.dumps({"config_small": small_line}))
             return
     cpu = None if args.no_cpu_baseline else cpu_large(info, nvis_total)
+    gram = None
+    if not args.no_gram:
+        gram = gram_solve(dev)
+        progress(f"Gram solve: CG {gram['cg_applies']} applies, learned {gram['learned_applies']}")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -354,6 +411,7 @@ def main():
         "per_call": per_call,
         "cpu_baseline": cpu,
         "config_small": small_line,
+        "gram_solve": gram,
     }
     print(json.dumps(line))
 
