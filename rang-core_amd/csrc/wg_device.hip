@@ -917,6 +917,12 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
         trv.load(coef, cv[0]);
         trw.load(coef, min(local, S - 1));
     }
+    // One visibility per wave (the w-gridding fast path): lane L evaluates
+    // tap L mod S of kind L / S, the u taps at x0, the v taps at y0 and the w
+    // taps of planes mp .. mp + S - 1; the u-v lanes fetch theirs across lanes.
+    const int tk = lane % S, kind = lane / S;
+    TapReg<T, S> trl;
+    if constexpr (REG && RPW == 1) trl.load(coef, tk);
     const bool stager = threadIdx.x < kGridBatch;
     Rec<T> nxt{};
     if (stager && jbeg + threadIdx.x < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x);
@@ -950,7 +956,13 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                     } else {
                         val[0] = {uni(rc_.re), uni(rc_.im)};
                     }
-                    const GridAcc kd = GridAcc(tru.eval(x0) * trv.eval(y0));
+                    // the w argument of tap tk (plane mp + tk), as w_factor forms it
+                    const double w0 = g.wmin + double(mp + tk) * g.dw;
+                    const T z = T(g.xdw * (w0 - w));
+                    const T warg = (z - T(tk)) * T(2) + T(double(S) - 1.0);
+                    const T tval = trl.eval(kind == 0 ? x0 : kind == 1 ? y0 : warg);
+                    const T ku = __shfl(tval, cu[0], 64), kv = __shfl(tval, S + cv[0], 64);
+                    const GridAcc kd = GridAcc(ku * kv);
                     const int iu = ou + cu[0], iv = ov + cv[0];
                     const bool inside = local < SS && unsigned(iu) < unsigned(g.Tu) &&
                                         unsigned(iv) < unsigned(g.Tv);
@@ -961,15 +973,9 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                         a[o] = GridAcc(val[o].re) * kd;
                         b[o] = GridAcc(val[o].im) * kd;
                     }
-                    // lane `local` < S evaluates the w tap `local` (plane mp + local),
-                    // as w_factor does
-                    const int nth = min(local, S - 1);
-                    const double w0 = g.wmin + double(mp + nth) * g.dw;
-                    const T z = T(g.xdw * (w0 - w));
-                    const T kwv = trw.eval((z - T(nth)) * T(2) + T(double(S) - 1.0));
                     GridAcc kw[S];
 #pragma unroll
-                    for (int t = 0; t < S; ++t) kw[t] = GridAcc(lane_value(kwv, t));
+                    for (int t = 0; t < S; ++t) kw[t] = GridAcc(lane_value(tval, 2 * S + t));
                     if (inside) {
 #pragma unroll
                         for (int t = 0; t < S; ++t) {
