@@ -19,7 +19,10 @@ workload and scaled to it; the same measurements on the small workload
 (3,225,600 visibilities, 1024 x 1024 at 2 arcsec) under "config_small"; and,
 under "gram_solve", the Gram applications plain CG and the learned
 preconditioner need to reach a relative residual of 1e-3 on a reduced
-certified-imaging case (--no-gram skips it).
+certified-imaging case (--no-gram skips it); under "published", the
+exact gridder and the w-stacking gridder on a 23.8 M-visibility 1024^2 case
+beside the reference's published timings for that size (--no-published
+skips it).
 """
 
 import argparse
@@ -256,6 +259,97 @@ def cpu_large(info, nvis_total):
     }
 
 
+def published(dev):
+    """The reference's own published timings at one size, beside this GPU's
+    2016 baselines x 738 times x 16 channels =
+    23,804,928 visibilities (the reference's whole-array cases have
+    23,811,985), 1024^2 at 1.5 arcsec.
+
+    * exact gridder, dirty image and PSF in one call, float64, epsilon 1e-6:
+      1.93 s per image on 16 threads (ducc0 1.81 s), research/df2-wgridder-
+      speed.md:290-291, 333-335; here the device call on HBM-resident inputs
+      and the host-array C entry point (PCIe included);
+    * w-stacking, 64 layers, kernel 7 x 1023, padding 1.2, dirty + PSF:
+      2.55 s (f64) and 1.96 s (f32) per image on the reference's GPU path,
+      wall including host preparation and transfers (research/x2-wgpu.md:
+      421-424); here rang_core_grid_invert_gpu from host arrays, the same
+      boundary."""
+    import numpy as np
+    import torch
+
+    from rang import _core, synth
+    from rang import device as D
+
+    c = synth.imaging_case_device(738, 16, 1.30e9, 1.45e9, 1024, 1.5, dev)
+    nrow, nchan = int(c["uvw"].shape[0]), int(c["freq"].shape[0])
+    pix = c["pixsize"]
+    opts = [1024, 1024, pix, pix, 1e-6, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    wgt64 = c["wgt"].to(torch.float64)
+    vis128 = c["vis"].to(torch.complex128)
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, info = D.vis2dirty(opts, c["uvw"], c["freq"], vis128, wgt64, single=False, psf=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_dev = min(times[1:])
+    # the host-array entry point (rang._core, as the reference's)
+    uvw_h = c["uvw"].cpu().numpy()
+    freq_h = c["freq"].cpu().numpy()
+    vis_h = vis128.cpu().numpy()
+    wgt_h = wgt64.cpu().numpy()
+    dirty_h = np.zeros((1024, 1024))
+    psf_h = np.zeros((1024, 1024))
+    info_h = np.zeros(_core.N_INFO)
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        st = _core.wgridder_vis2dirty_f64(opts, uvw_h, freq_h, vis_h, wgt_h, None, None, dirty_h, psf_h,
+                                          info_h)
+        times.append(time.perf_counter() - t0)
+        if st != 0:
+            raise RuntimeError(_core.wgridder_last_error())
+    t_host = times[-1]
+    progress(f"exact gridder f64 dirty+PSF: device {t_dev:.3f} s, host arrays {t_host:.3f} s")
+    data = np.ascontiguousarray(c["vis"].cpu().numpy().view(np.float32).reshape(nrow, nchan, 2))
+    w32 = np.ascontiguousarray(c["wgt"].cpu().numpy().astype(np.float32))
+    ws = {}
+    for prec in ("f64", "f32"):
+        d = np.zeros(1024 * 1024)
+        p = np.zeros(1024 * 1024)
+        stats = np.zeros(_core.GRID_N_STATS)
+        timings = np.zeros(_core.GRID_N_TIMINGS)
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            st = _core.grid_invert_gpu(nrow, nchan, 1024, 1024, 7, 1023, 64, pix, pix, 1.2, uvw_h, freq_h, data,
+                                       w32, w32, prec, d, p, stats, timings)
+            t = time.perf_counter() - t0
+            if st != 0:
+                raise RuntimeError(_core.grid_last_error())
+            best = t if best is None else min(best, t)
+        ws[prec] = round(best, 3)
+        progress(f"w-stacking {prec}: {best:.3f} s")
+    del c, vis128, wgt64
+    torch.cuda.empty_cache()
+    return {
+        "workload": "2016 baselines x 738 times x 16 channels (23,804,928 vis, 1.30-1.45 GHz), "
+                    "1024x1024 at 1.5 arcsec, dirty + PSF",
+        "wgridder_f64_eps1e-6": {
+            "grid": [int(info[2]), int(info[3])], "support": int(info[1]), "w_planes": int(info[4]),
+            "seconds_device": round(t_dev, 3), "seconds_host_arrays": round(t_host, 3),
+            "reference_seconds_16_threads": 1.93, "ducc0_seconds_16_threads": 1.81,
+            "speedup_host_arrays": round(1.93 / t_host, 1),
+        },
+        "wstack_64_layers": {
+            "seconds_f64": ws["f64"], "seconds_f32": ws["f32"],
+            "reference_gpu_seconds_f64": 2.55, "reference_gpu_seconds_f32": 1.96,
+            "speedup_f64": round(2.55 / ws["f64"], 1), "speedup_f32": round(1.96 / ws["f32"], 1),
+        },
+    }
+
+
 def gram_solve(dev, tol=1e-3, budget=2000, steps=400):
     """The Gram-solve half of the metric on a reduced certified-imaging case
     (2016 baselines x 32 times x 8 channels, 128^2 at 2 arcsec, float64,
@@ -314,6 +408,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the small workload")
     ap.add_argument("--no-gram", action="store_true", help="skip the Gram-solve comparison")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the cases the reference published timings for")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -378,6 +474,7 @@ def main():
             print(json.dumps({"config_small": small_line}))
             return
     cpu = None if args.no_cpu_baseline else cpu_large(info, nvis_total)
+    pub = None if args.no_published else published(dev)
     gram = None
     if not args.no_gram:
         gram = gram_solve(dev)
@@ -412,6 +509,7 @@ def main():
         "cpu_baseline": cpu,
         "config_small": small_line,
         "gram_solve": gram,
+        "published": pub,
     }
     print(json.dumps(line))
 
