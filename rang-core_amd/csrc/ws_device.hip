@@ -16,6 +16,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -199,19 +201,35 @@ __global__ __launch_bounds__(256) void k_keys(In in, Geo g, int* keys, double* p
     }
 }
 
-__global__ __launch_bounds__(256) void k_hist(const int* keys, long long n, unsigned* counts) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        if (keys[i] >= 0) atomicAdd(&counts[keys[i]], 1u);
+// The counting sort by layer in contiguous chunks, one block each: per
+// block the histogram of its chunk's layers in LDS (no global atomics on the
+// few layer counters), and the scatter through LDS cursors that start at the
+// block's offsets within each layer (chunks in order; within a chunk the
+// order of the LDS atomics).
+__global__ __launch_bounds__(256) void k_hist_chunks(const int* keys, long long n, long long chunk, int L,
+                                                     unsigned* blockcounts) {
+    extern __shared__ unsigned hist[];
+    for (int l = threadIdx.x; l < L; l += 256) hist[l] = 0;
+    __syncthreads();
+    const long long a = (long long)blockIdx.x * chunk, b = a + chunk < n ? a + chunk : n;
+    for (long long i = a + threadIdx.x; i < b; i += 256)
+        if (keys[i] >= 0) atomicAdd(&hist[keys[i]], 1u);
+    __syncthreads();
+    for (int l = threadIdx.x; l < L; l += 256) blockcounts[(long long)blockIdx.x * L + l] = hist[l];
 }
 
-__global__ __launch_bounds__(256) void k_scatter(In in, Geo g, const int* keys, const unsigned* off,
-                                                 unsigned* cursor, Sample* out) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < in.n; i += (long long)gridDim.x * 256) {
+__global__ __launch_bounds__(256) void k_scatter_chunks(In in, Geo g, const int* keys, long long chunk, int L,
+                                                        const unsigned* blockoff, Sample* out) {
+    extern __shared__ unsigned cur[];
+    for (int l = threadIdx.x; l < L; l += 256) cur[l] = blockoff[(long long)blockIdx.x * L + l];
+    __syncthreads();
+    const long long a = (long long)blockIdx.x * chunk, b = a + chunk < in.n ? a + chunk : in.n;
+    for (long long i = a + threadIdx.x; i < b; i += 256) {
         if (keys[i] < 0) continue;
         Sample s;
         float weight, cw;
         place(in, g, i, s, weight, cw);
-        out[off[s.layer] + atomicAdd(&cursor[s.layer], 1u)] = s;
+        out[atomicAdd(&cur[s.layer], 1u)] = s;
     }
 }
 
@@ -359,19 +377,45 @@ void set_lds(K kernel, std::size_t bytes) {
 }
 
 // Device buffers of one inversion, freed on every exit.
+// Device buffers kept per device between calls (allocating and freeing
+// gigabytes per call cost more than the gridding): the k-th request of a
+// call reuses the k-th slot, grown when too small. Calls on one device are
+// serialised by the pool's mutex.
+struct Pool {
+    std::mutex mu;
+    std::vector<std::pair<void*, std::size_t>> slots;
+};
+
+Pool& pool_of_current_device() {
+    static std::mutex mu;
+    static std::map<int, Pool*> pools;
+    int dev = 0;
+    WSCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    Pool*& p = pools[dev];
+    if (!p) p = new Pool;  // kept until the process ends, as the buffers are
+    return *p;
+}
+
 struct Bufs {
-    std::vector<void*> ptrs;
+    Pool& pool;
+    std::size_t next = 0;
     std::size_t bytes = 0;
+    explicit Bufs(Pool& p) : pool(p) {}
     template <class P>
     P* get(std::size_t n) {
-        void* p = nullptr;
-        WSCHK(hipMalloc(&p, std::max<std::size_t>(n * sizeof(P), 16)));
-        ptrs.push_back(p);
+        const std::size_t need = std::max<std::size_t>(n * sizeof(P), 16);
+        if (next >= pool.slots.size()) pool.slots.push_back({nullptr, 0});
+        auto& s = pool.slots[next++];
+        if (s.second < need) {
+            if (s.first) WSCHK(hipFree(s.first));
+            s.first = nullptr;
+            s.second = 0;
+            WSCHK(hipMalloc(&s.first, need));
+            s.second = need;
+        }
         bytes += n * sizeof(P);
-        return static_cast<P*>(p);
-    }
-    ~Bufs() {
-        for (void* p : ptrs) (void)hipFree(p);
+        return static_cast<P*>(s.first);
     }
 };
 
@@ -426,7 +470,9 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{st};
-    Bufs b;
+    Pool& pool = pool_of_current_device();
+    std::lock_guard<std::mutex> pool_lock(pool.mu);
+    Bufs b(pool);
     // upload
     auto t0 = Clock::now();
     double* d_uvw = b.get<double>(3 * nrow);
@@ -464,12 +510,15 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     k_keys<<<nblk, 256, 0, st>>>(in, g, d_keys, d_part);
     WSCHK(hipGetLastError());
     WSCHK(hipMemcpyAsync(part.data(), d_part, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, st));
-    unsigned* d_cnt = b.get<unsigned>(2 * (i.n_w_layers + 1));
-    WSCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned) * 2 * (i.n_w_layers + 1), st));
-    k_hist<<<nblk, 256, 0, st>>>(d_keys, (long long)n, d_cnt);
+    const int L = int(i.n_w_layers);
+    const int nch = 1024;
+    const long long chunk = ((long long)n + nch - 1) / nch;
+    unsigned* d_bc = b.get<unsigned>(std::size_t(nch) * L);
+    if (std::size_t(L) * sizeof(unsigned) > 64 * 1024) throw Fail{1, "too many w-layers for the GPU sort"};
+    k_hist_chunks<<<nch, 256, sizeof(unsigned) * L, st>>>(d_keys, (long long)n, chunk, L, d_bc);
     WSCHK(hipGetLastError());
-    std::vector<unsigned> cnt(i.n_w_layers + 1);
-    WSCHK(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned) * i.n_w_layers, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned> bc(std::size_t(nch) * L);
+    WSCHK(hipMemcpyAsync(bc.data(), d_bc, sizeof(unsigned) * bc.size(), hipMemcpyDeviceToHost, st));
     WSCHK(hipStreamSynchronize(st));
     double image_weight = 0, vsum = 0, gridded = 0, maxw = 0;
     for (int k = 0; k < nblk; ++k) {
@@ -479,14 +528,24 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
         maxw = std::max(maxw, part[4 * k + 3]);
     }
     if (image_weight == 0.0) throw Fail{1, "no visibility carries weight"};
-    std::vector<unsigned> off(i.n_w_layers + 1, 0);
-    for (std::size_t l = 0; l < i.n_w_layers; ++l) off[l + 1] = off[l] + cnt[l];
-    const std::size_t ns = off[i.n_w_layers];
-    unsigned* d_off = b.get<unsigned>(i.n_w_layers + 1);
-    unsigned* d_cur = d_cnt + (i.n_w_layers + 1);
+    std::vector<unsigned> cnt(std::size_t(L) + 1, 0), off(std::size_t(L) + 1, 0);
+    for (int c = 0; c < nch; ++c)
+        for (int l = 0; l < L; ++l) cnt[l] += bc[std::size_t(c) * L + l];
+    for (int l = 0; l < L; ++l) off[l + 1] = off[l] + cnt[l];
+    // block c's first slot in layer l: the layer's offset plus the earlier chunks' counts
+    {
+        std::vector<unsigned> run(off.begin(), off.end() - 1);
+        for (int c = 0; c < nch; ++c)
+            for (int l = 0; l < L; ++l) {
+                const unsigned k = bc[std::size_t(c) * L + l];
+                bc[std::size_t(c) * L + l] = run[l];
+                run[l] += k;
+            }
+    }
+    const std::size_t ns = off[L];
     Sample* d_s = b.get<Sample>(std::max<std::size_t>(ns, 1));
-    WSCHK(hipMemcpyAsync(d_off, off.data(), sizeof(unsigned) * (i.n_w_layers + 1), hipMemcpyHostToDevice, st));
-    k_scatter<<<nblk, 256, 0, st>>>(in, g, d_keys, d_off, d_cur, d_s);
+    WSCHK(hipMemcpyAsync(d_bc, bc.data(), sizeof(unsigned) * bc.size(), hipMemcpyHostToDevice, st));
+    k_scatter_chunks<<<nch, 256, sizeof(unsigned) * L, st>>>(in, g, d_keys, chunk, L, d_bc, d_s);
     WSCHK(hipGetLastError());
     WSCHK(hipStreamSynchronize(st));
     t_bucket = since(t0);
