@@ -31,6 +31,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "wg_device.h"
@@ -1181,6 +1182,53 @@ __device__ __forceinline__ cx<T> screen_at(double w, double s, bool adjoint) {
     return {cs, sn};
 }
 
+// Fills LDS from memory with every load of the block in flight before the
+// first LDS write (a transform's loads are otherwise serialised on their
+// latency): element e < count (count <= MT) is src[at(e).first], stored at
+// buf[at(e).second].
+template <class T, int NT, int MT, class At>
+__device__ __forceinline__ void staged_fill(cx<T>* buf, const cx<T>* src, int count, At at) {
+    constexpr int PER = (MT + NT - 1) / NT;
+    if constexpr (sizeof(T) == 4) {
+        cx<T> r[PER];
+        int dst[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if (e < count) {
+                const auto sd = at(e);
+                r[k] = src[sd.first];
+                dst[k] = sd.second;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if (e < count) buf[dst[k]] = r[k];
+        }
+    } else {
+        // double: the values staged as two-lane vectors (staging arrays of the
+        // complex struct are placed in scratch memory by the compiler)
+        using V = __attribute__((ext_vector_type(2))) double;
+        V r[PER];
+        int dst[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if (e < count) {
+                const auto sd = at(e);
+                r[k] = *reinterpret_cast<const V*>(src + sd.first);
+                dst[k] = sd.second;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if (e < count) *reinterpret_cast<V*>(buf + dst[k]) = r[k];
+        }
+    }
+}
+
 // A linear block index remapped so that the workgroups one XCD holds at a time
 // take consecutive indices: the dispatcher deals consecutive workgroups to the
 // eight XCDs in turn, and each XCD has its own L2, so blocks that read
@@ -1203,19 +1251,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
     const int nv = d.n;
     const long long row = blockIdx.x;
     const cx<T>* src = grid + row * nv;
-    // every load of the row in flight before the first LDS write
-    constexpr int PER = (MAXTOT + NT - 1) / NT;
-    cx<T> r[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < nv) r[k] = src[e];
-    }
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (e < nv) buf[e] = r[k];
-    }
+    staged_fill<T, NT, MAXTOT>(buf, src, nv, [](int e) { return std::pair<long long, int>(e, e); });
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, +1);
     const int sh = nv - ny / 2;
@@ -1242,10 +1278,11 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
     const int j0 = (lin - p * ncb) * C, nc = min(C, ny - j0);
     const cx<T>* src = tmp + (long long)p * nu * ny;
     const unsigned mnc = nc > 1 ? unsigned(((1ull << 32) + nc - 1) / nc) : 0u;
-    for (int e = threadIdx.x; e < nu * nc; e += NT) {
+    // nu * nc <= MAXTOT
+    staged_fill<T, NT, MAXTOT>(buf, src, nu * nc, [&](int e) {
         const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
-        buf[c * nu + u] = src[(long long)u * ny + j0 + c];
-    }
+        return std::pair<long long, int>((long long)u * ny + j0 + c, c * nu + u);
+    });
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, +1);
     const double w0 = g.wmin + double(p) * g.dw;
@@ -1515,10 +1552,10 @@ __global__ __launch_bounds__(NT) void k_lrow_p1(cx<T>* data, int n1, int n2, int
     const long long row = blockIdx.x / nb;
     const int j20 = int(blockIdx.x - row * nb) * C2, nc = min(C2, n2 - j20);
     cx<T>* x = data + row * (long long)n1 * n2;
-    for (int e = threadIdx.x; e < n1 * nc; e += NT) {
+    staged_fill<T, NT, MT>(buf, x, n1 * nc, [&](int e) {
         const int j1 = e / nc, c = e - j1 * nc;
-        buf[c * n1 + j1] = x[(long long)j1 * n2 + j20 + c];
-    }
+        return std::pair<long long, int>((long long)j1 * n2 + j20 + c, c * n1 + j1);
+    });
     __syncthreads();
     fft_lds<T, NT, MT>(buf, d1, nc, tw1, sgn);
     for (int e = threadIdx.x; e < n1 * nc; e += NT) {
@@ -1539,7 +1576,8 @@ __global__ __launch_bounds__(NT) void k_lrow_p2(cx<T>* data, int n1, int n2, int
     const int k10 = int(blockIdx.x - row * nb) * K1, nc = min(K1, n1 - k10);
     const int n = n1 * n2;
     cx<T>* x = data + row * (long long)n;
-    for (int e = threadIdx.x; e < n2 * nc; e += NT) buf[e] = x[(long long)k10 * n2 + e];
+    staged_fill<T, NT, MT>(buf, x, n2 * nc,
+                           [&](int e) { return std::pair<long long, int>((long long)k10 * n2 + e, e); });
     __syncthreads();
     fft_lds<T, NT, MT>(buf, d2, nc, tw2, sgn);
     for (int e = threadIdx.x; e < n2 * nc; e += NT) {
@@ -1568,10 +1606,10 @@ __global__ __launch_bounds__(NT) void k_lcol_p1(cx<T>* data, int n1, int n2, int
     const int m0 = mb * Cm, ncm = min(Cm, M - m0), j20 = jb * C2, nc2 = min(C2, n2 - j20);
     const int lines = ncm * nc2;
     cx<T>* x = data + (long long)p * n1 * n2 * M;
-    for (int e = threadIdx.x; e < n1 * lines; e += NT) {
+    staged_fill<T, NT, MT>(buf, x, n1 * lines, [&](int e) {
         const int j1 = e / lines, r = e - j1 * lines, c2 = r / ncm, cm = r - c2 * ncm;
-        buf[r * n1 + j1] = x[((long long)j1 * n2 + j20 + c2) * M + m0 + cm];
-    }
+        return std::pair<long long, int>(((long long)j1 * n2 + j20 + c2) * M + m0 + cm, r * n1 + j1);
+    });
     __syncthreads();
     fft_lds<T, NT, MT>(buf, d1, lines, tw1, sgn);
     for (int e = threadIdx.x; e < n1 * lines; e += NT) {
@@ -1596,10 +1634,10 @@ __global__ __launch_bounds__(NT) void k_lcol_p2(cx<T>* data, int n1, int n2, int
     const int m0 = mb * Cm, ncm = min(Cm, M - m0), k10 = kb * K1, nc1 = min(K1, n1 - k10);
     const int lines = ncm * nc1, n = n1 * n2;
     cx<T>* x = data + (long long)p * n * M;
-    for (int e = threadIdx.x; e < n2 * lines; e += NT) {
+    staged_fill<T, NT, MT>(buf, x, n2 * lines, [&](int e) {
         const int j2 = e / lines, r = e - j2 * lines, c1 = r / ncm, cm = r - c1 * ncm;
-        buf[r * n2 + j2] = x[((long long)(k10 + c1) * n2 + j2) * M + m0 + cm];
-    }
+        return std::pair<long long, int>(((long long)(k10 + c1) * n2 + j2) * M + m0 + cm, r * n2 + j2);
+    });
     __syncthreads();
     fft_lds<T, NT, MT>(buf, d2, lines, tw2, sgn);
     const double w0 = g.wmin + double(p) * g.dw;

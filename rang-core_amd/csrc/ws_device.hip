@@ -348,6 +348,16 @@ __global__ __launch_bounds__(256) void k_finish(const double* sd, const double* 
     op[e] = (sp[s] * factor) / div;
 }
 
+// dct3_prefix (grid.rs:263-279): out[k] = in[0] + 2 sum_j in[j] cos(pi j (k + 1/2) / n),
+// one block per k.
+__global__ __launch_bounds__(256) void k_dct3(const double* in, int len, double n, double* out) {
+    const double kk = double(blockIdx.x) + 0.5;
+    double s = 0.0, unused = 0.0;
+    for (int j = 1 + int(threadIdx.x); j < len; j += 256) s += 2.0 * in[j] * cos(M_PI * double(j) * kk / n);
+    block_reduce2(s, unused, false);
+    if (threadIdx.x == 0) out[blockIdx.x] = in[0] + s;
+}
+
 template <int V>
 using ic = std::integral_constant<int, V>;
 
@@ -436,18 +446,6 @@ double bessel0(double x, double precision) {
         if (ds <= sum * precision) break;
     }
     return sum;
-}
-
-// dct3_prefix (grid.rs:263-279).
-std::vector<double> dct3(const double* in, std::size_t len, std::size_t n, std::size_t count) {
-    std::vector<double> out(count);
-    for (std::size_t k = 0; k < count; ++k) {
-        const double kk = double(k) + 0.5;
-        double s = in[0];
-        for (std::size_t j = 1; j < len; ++j) s += 2.0 * in[j] * std::cos(M_PI * double(j) * kk / double(n));
-        out[k] = s;
-    }
-    return out;
 }
 
 template <class T>
@@ -647,13 +645,17 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     // normalise, correct, trim (grid.rs:597-650, 837-873)
     t0 = Clock::now();
     const std::size_t half = klen / 2, rlen = klen / 2 + 1;
-    std::vector<double> ox = dct3(k1.data() + half, rlen, W * os / 2, W / 2 + 1);
-    std::vector<double> oy = i.width == i.height ? dct3(k1.data() + half, rlen, H * os / 2, H / 2 + 1)
-                                                 : dct3(k1.data() + half, rlen, H * os / 2, H / 2 + 1);
-    double* d_ox = b.get<double>(ox.size());
-    double* d_oy = b.get<double>(oy.size());
-    WSCHK(hipMemcpyAsync(d_ox, ox.data(), sizeof(double) * ox.size(), hipMemcpyHostToDevice, st));
-    WSCHK(hipMemcpyAsync(d_oy, oy.data(), sizeof(double) * oy.size(), hipMemcpyHostToDevice, st));
+    double* d_k1 = b.get<double>(rlen);
+    WSCHK(hipMemcpyAsync(d_k1, k1.data() + half, sizeof(double) * rlen, hipMemcpyHostToDevice, st));
+    double* d_ox = b.get<double>(W / 2 + 1);
+    k_dct3<<<int(W / 2 + 1), 256, 0, st>>>(d_k1, int(rlen), double(W * os / 2), d_ox);
+    WSCHK(hipGetLastError());
+    double* d_oy = d_ox;
+    if (H != W) {
+        d_oy = b.get<double>(H / 2 + 1);
+        k_dct3<<<int(H / 2 + 1), 256, 0, st>>>(d_k1, int(rlen), double(H * os / 2), d_oy);
+        WSCHK(hipGetLastError());
+    }
     const std::size_t npix = i.width * i.height;
     double* d_od = b.get<double>(npix);
     double* d_op = b.get<double>(npix);
