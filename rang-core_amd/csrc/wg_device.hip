@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -836,11 +837,39 @@ using GridAcc = double;
 
 // Record j of an item's concatenated source ranges, with its in-tile offsets
 // made relative to the item's tile (biased by 64: tiles are at most 64 wide,
-// so the offsets lie in [-64, 64)).
+// so the offsets lie in [-64, 64)). With w-gridding its w becomes the
+// argument of the w kernel's taps, formed once per record here (one lane per
+// record) rather than per tap in the gridding loop (one wave per record).
+// w_factor's argument (z - nth) 2 + S - 1 with z = xdw (wmin + (mp + nth) dw
+// - w) is, as xdw = 1 / dw, the same for every tap nth:
+// 2 xdw (wmin + mp dw - w) + S - 1 (wgridder.rs:1580-1591, 696-698).
+// Its first plane mp becomes, for the item's planes p0 .. p1 - 1, the range
+// [tlo, thi) of its taps on them and the item plane q0 of tap tlo (5 bits
+// each above the offsets; mp = p0 + q0 - tlo).
+// The w taps' argument of a staged record, in the T just before x0 (the
+// upper half of w in single precision, w itself in double), so that the
+// arguments of the u, v and w taps lie at x0, x0 + 1 and x0 - 1.
+template <class T>
+__device__ __forceinline__ T* warg_slot(Rec<T>& r) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(&r) + offsetof(Rec<T>, x0) - sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T warg_of(const Rec<T>& r) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(&r) + offsetof(Rec<T>, x0) - sizeof(T));
+}
+
+__device__ __forceinline__ void taps_on_item(std::uint32_t pos, int p0, int& tlo, int& thi, int& q0, int& mp) {
+    tlo = int((pos >> 16) & 31u);
+    thi = int((pos >> 21) & 31u);
+    q0 = int((pos >> 26) & 31u);
+    mp = p0 + q0 - tlo;
+}
+
 template <class T>
 __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restrict__ recs,
                                              const std::uint32_t* s_start,
-                                             const std::uint32_t* s_pre, std::uint32_t j) {
+                                             const std::uint32_t* s_pre, std::uint32_t j, int p0,
+                                             int p1) {
     int src = 0;
     while (j >= s_pre[src + 1]) ++src;
     Rec<T> r = recs[s_start[src] + (j - s_pre[src])];
@@ -848,7 +877,13 @@ __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restr
     source_of(src, du, dv, cat);
     const int ou = int(r.pos & 255u) - (du ? g.Tu : 0) + 64;
     const int ov = int((r.pos >> 8) & 255u) - (dv ? g.Tv : 0) + 64;
-    r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8) | (r.pos & 0xffff0000u);
+    const int mp = int(r.pos >> 16);
+    r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8);
+    if (g.wgrid) {
+        *warg_slot(r) = T(2.0 * g.xdw * (g.wmin + double(mp) * g.dw - r.w) + (double(g.supp) - 1.0));
+        const int tlo = max(0, p0 - mp), thi = min(g.supp, p1 - mp);
+        r.pos |= (std::uint32_t(tlo) << 16) | (std::uint32_t(thi) << 21) | (std::uint32_t(mp + tlo - p0) << 26);
+    }
     return r;
 }
 
@@ -924,70 +959,80 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
     const int tk = lane % S, kind = lane / S;
     TapReg<T, S> trl;
     if constexpr (REG && RPW == 1) trl.load(coef, tk);
+    // the lane's tap argument: x0 (u), x0 + 1 (y0, v) or x0 - 1 (w), in T
+    // units from x0; the lane's cell offset from the visibility's first cell
+    const int argoff = kind == 0 ? 0 : kind == 1 ? 1 : kind == 2 ? -1 : 0;
+    unsigned lane8 = unsigned(cu[0] * TvP + cv[0]) * unsigned(sizeof(GridAcc));
+    // opaque to the optimiser, which would otherwise fold it into a per-visit
+    // (cu + ou) TvP multiply
+    asm volatile("" : "+v"(lane8));
+    const unsigned pstride8 = unsigned(pstride) * unsigned(sizeof(GridAcc));
     const bool stager = threadIdx.x < kGridBatch;
     Rec<T> nxt{};
-    if (stager && jbeg + threadIdx.x < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x);
+    if (stager && jbeg + threadIdx.x < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x, p0, p1);
     for (std::uint32_t base = jbeg; base < jend; base += kGridBatch) {
         const int nbat = int(min(jend - base, std::uint32_t(kGridBatch)));
         __syncthreads();  // the previous batch is consumed
         if (int(threadIdx.x) < nbat) s_rec[threadIdx.x] = nxt;
         __syncthreads();
         const std::uint32_t jn = base + kGridBatch + threadIdx.x;
-        if (stager && jn < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jn);
+        if (stager && jn < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jn, p0, p1);
         if constexpr (REG && RPW == 1) {
             if (g.wgrid) {
-                // One visibility per wave: its fields are wave-uniform, the
-                // plane tests scalar, and the w taps are read from lanes
-                // 0..S-1 into scalar registers.
-                for (int r = wave; r < nbat; r += NW) {
+                // One visibility per wave: its position is wave-uniform, and
+                // only its taps on the item's planes are visited, each w tap
+                // read from its lane. The loops are scalar: the scalar unit
+                // (one instruction per SIMD every four clocks) bounds this
+                // loop before the vector unit or the LDS does.
+                const int wv = __builtin_amdgcn_readfirstlane(wave);
+                for (int r = wv; r < nbat; r += NW) {
                     const Rec<T>& rc_ = s_rec[r];
+                    // the position wave-uniform (scalar control of the plane
+                    // loop); the other fields are broadcast reads in every lane
                     const std::uint32_t pos = uni(rc_.pos);
                     const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
-                    const int mp = int(pos >> 16);
-                    const T x0 = uni(rc_.x0), y0 = uni(rc_.y0);
-                    const double w = uni(rc_.w);
+                    int tlo, thi, q0, mp;
+                    taps_on_item(pos, p0, tlo, thi, q0, mp);
+                    const T arg = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(&rc_) +
+                                                              offsetof(Rec<T>, x0) + argoff * int(sizeof(T)));
                     cx<T> val[NOUT];
                     if constexpr (NOUT == 2 || FORM) {
                         const std::uint32_t idx = uni(rc_.idx);
-                        const T re = uni(rc_.re), im = uni(rc_.im);
+                        const T re = rc_.re, im = rc_.im;
                         const T fl = (idx & kFlip) ? T(-1) : T(1);
                         const cx<T> d = cmul(vis_get<T>(in, idx & ~kFlip), cx<T>{re, im * fl});
                         val[0] = {d.re, d.im * fl};
                         if constexpr (NOUT == 2) val[1] = {re, im};
                     } else {
-                        val[0] = {uni(rc_.re), uni(rc_.im)};
+                        val[0] = {rc_.re, rc_.im};
                     }
-                    // the w argument of tap tk (plane mp + tk), as w_factor forms it
-                    const double w0 = g.wmin + double(mp + tk) * g.dw;
-                    const T z = T(g.xdw * (w0 - w));
-                    const T warg = (z - T(tk)) * T(2) + T(double(S) - 1.0);
-                    const T tval = trl.eval(kind == 0 ? x0 : kind == 1 ? y0 : warg);
+                    const T tval = trl.eval(arg);
                     const T ku = __shfl(tval, cu[0], 64), kv = __shfl(tval, S + cv[0], 64);
                     const GridAcc kd = GridAcc(ku * kv);
                     const int iu = ou + cu[0], iv = ov + cv[0];
                     const bool inside = local < SS && unsigned(iu) < unsigned(g.Tu) &&
                                         unsigned(iv) < unsigned(g.Tv);
-                    const int at = iu * TvP + iv;
                     GridAcc a[NOUT], b[NOUT];
 #pragma unroll
                     for (int o = 0; o < NOUT; ++o) {
                         a[o] = GridAcc(val[o].re) * kd;
                         b[o] = GridAcc(val[o].im) * kd;
                     }
-                    GridAcc kw[S];
-#pragma unroll
-                    for (int t = 0; t < S; ++t) kw[t] = GridAcc(lane_value(tval, 2 * S + t));
+                    // taps t in [tlo, thi) fall on the item's planes q0, q0 + 1, ...
                     if (inside) {
-#pragma unroll
-                        for (int t = 0; t < S; ++t) {
-                            const int p = mp + t;
-                            if (p < p0 || p >= p1) continue;
+                        // byte offsets: the lane's cell, plus the wave-uniform
+                        // first cell and plane
+                        unsigned base8 = unsigned((q0 * 2 * pstride + ou * TvP + ov)) * unsigned(sizeof(GridAcc));
+                        for (int t = 2 * S + tlo; t < 2 * S + thi; ++t) {
+                            const GridAcc kw = GridAcc(lane_value(tval, t));
+                            const unsigned re8 = lane8 + base8;
 #pragma unroll
                             for (int o = 0; o < NOUT; ++o) {
-                                GridAcc* are = acc + ((o * np + (p - p0)) * 2) * pstride + at;
-                                atomicAdd(are, a[o] * kw[t]);
-                                atomicAdd(are + pstride, b[o] * kw[t]);
+                                const unsigned oo = unsigned(o * np * 2) * pstride8;
+                                atomicAdd(reinterpret_cast<GridAcc*>(smem + re8 + oo), a[o] * kw);
+                                atomicAdd(reinterpret_cast<GridAcc*>(smem + re8 + oo + pstride8), b[o] * kw);
                             }
+                            base8 += 2 * pstride8;
                         }
                     }
                 }
@@ -1000,7 +1045,8 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
             const Rec<T>& rc_ = s_rec[valid ? r : rw];
             const std::uint32_t pos = rc_.pos;
             const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
-            const int mp = int(pos >> 16);
+            int tlo, thi, q0, mp;
+            taps_on_item(pos, p0, tlo, thi, q0, mp);
             const T x0 = rc_.x0, y0 = rc_.y0;
             cx<T> val[NOUT];
             if constexpr (NOUT == 2 || FORM) {
@@ -1040,14 +1086,11 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                 // lane `local` < S evaluates the w tap `local` (plane mp + local)
                 const int nth = min(local, S - 1);
                 T kwv;
-                if constexpr (REG) {
-                    // w_factor's argument with the lane's polynomial
-                    const double w0 = g.wmin + double(mp + nth) * g.dw;
-                    const T z = T(g.xdw * (w0 - rc_.w));
-                    kwv = trw.eval((z - T(nth)) * T(2) + T(double(S) - 1.0));
-                } else {
-                    kwv = w_factor<T, S>(g, coef, nth, mp + nth, rc_.w);
-                }
+                // the record holds the taps' argument (grid_fetch)
+                if constexpr (REG)
+                    kwv = trw.eval(warg_of(rc_));
+                else
+                    kwv = eval_tap<T, S>(coef, nth, warg_of(rc_));
 #pragma unroll
                 for (int nth2 = 0; nth2 < S; ++nth2) {
                     const GridAcc kw = GridAcc(__shfl(kwv, srclane + nth2, 64));
