@@ -850,10 +850,6 @@ using GridAcc = double;
 // upper half of w in single precision, w itself in double), so that the
 // arguments of the u, v and w taps lie at x0, x0 + 1 and x0 - 1.
 template <class T>
-__device__ __forceinline__ T* warg_slot(Rec<T>& r) {
-    return reinterpret_cast<T*>(reinterpret_cast<char*>(&r) + offsetof(Rec<T>, x0) - sizeof(T));
-}
-template <class T>
 __device__ __forceinline__ T warg_of(const Rec<T>& r) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(&r) + offsetof(Rec<T>, x0) - sizeof(T));
 }
@@ -880,7 +876,11 @@ __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restr
     const int mp = int(r.pos >> 16);
     r.pos = std::uint32_t(ou) | (std::uint32_t(ov) << 8);
     if (g.wgrid) {
-        *warg_slot(r) = T(2.0 * g.xdw * (g.wmin + double(mp) * g.dw - r.w) + (double(g.supp) - 1.0));
+        const T wa = T(2.0 * g.xdw * (g.wmin + double(mp) * g.dw - r.w) + (double(g.supp) - 1.0));
+        if constexpr (sizeof(T) == 4)  // the upper half of w (little-endian)
+            r.w = __longlong_as_double((long long)((unsigned long long)__float_as_uint(float(wa)) << 32));
+        else
+            r.w = double(wa);
         const int tlo = max(0, p0 - mp), thi = min(g.supp, p1 - mp);
         r.pos |= (std::uint32_t(tlo) << 16) | (std::uint32_t(thi) << 21) | (std::uint32_t(mp + tlo - p0) << 26);
     }
