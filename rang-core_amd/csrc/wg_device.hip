@@ -769,17 +769,19 @@ __global__ __launch_bounds__(256) void k_grid_items(Geo g, const std::uint32_t* 
         source_range(g, off, tu, tv, src, mlo, mhi, a, b);
         total += b - a;
     }
-    nitems[tg] = max(1u, (total + rc - 1) / rc);
+    // a unit without visibilities takes no item: k_zero_split clears its cells
+    nitems[tg] = (total + rc - 1) / rc;
 }
 
 // Cells of the (tile, plane group)s gridded by more than one item, zeroed
-// before those items add into them.
+// before those items add into them, and of those gridded by none (no
+// visibility reaches them).
 template <class T, int NOUT>
 __global__ __launch_bounds__(256) void k_zero_split(Geo g, const std::uint32_t* __restrict__ item_off,
                                                     const std::uint32_t* __restrict__ order, int G,
                                                     int ngroups, cx<T>* __restrict__ grid) {
     const int tg = blockIdx.x;
-    if (item_off[tg + 1] - item_off[tg] <= 1) return;
+    if (item_off[tg + 1] - item_off[tg] == 1) return;
     const int rank = tg / ngroups, grp = tg - rank * ngroups;
     const int tile = int(order[rank]);
     const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
@@ -2710,7 +2712,9 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     HIPCHK(hipGetLastError());
     const long long ub = read_u32(ws, item_off + ntg, st);
     if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
-    if (nout == 2)
+    if (ub == 0) {
+        // nothing to grid: every cell was cleared above
+    } else if (nout == 2)
         launch_grid<T, 2, false>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
                                  ngroups, TvP, rc, item_off, tb.order, grid);
     else if (form)
