@@ -1808,10 +1808,35 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
     const int P = g.wgrid ? g.nplanes : 1;
     const long long plane = (long long)g.nu * g.nv;
     const int SU = g.Tu + S - 1, SV = g.Tv + S - 1;
+    // With STAGE, the planes [plo, phi) the chunk's visibilities reach: their
+    // first planes are the bucket bins (tile, spill class, first plane) whose
+    // ranges of records overlap the chunk's [a, b).
+    int plo = 0, phi = P;
     if constexpr (STAGE) {
+        if (g.wgrid) {
+            __shared__ int s_lo, s_hi;
+            if (threadIdx.x == 0) {
+                s_lo = P;
+                s_hi = -1;
+            }
+            __syncthreads();
+            const long long k0 = (long long)tile * 4 * g.nb;
+            for (int k = threadIdx.x; k < 4 * g.nb; k += blockDim.x) {
+                const std::uint32_t ks = off[k0 + k], ke = off[k0 + k + 1];
+                if (ke > ks && ks < b && ke > a) {
+                    const int m = k - (k / g.nb) * g.nb;
+                    atomicMin(&s_lo, m);
+                    atomicMax(&s_hi, m);
+                }
+            }
+            __syncthreads();
+            plo = s_lo;
+            phi = min(P, s_hi + S);
+        }
         const int area = SU * SV;
-        for (int e = threadIdx.x; e < P * area; e += blockDim.x) {
-            const int p = e / area, rem = e - p * area;
+        for (int e = threadIdx.x; e < (phi - plo) * area; e += blockDim.x) {
+            const int pl = e / area, rem = e - pl * area;
+            const int p = plo + pl;
             const int i = rem / SV, j = rem - (rem / SV) * SV;
             const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
             const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
@@ -1847,7 +1872,7 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
                 const T kuc = g.wgrid ? ku[cu] * kw : ku[cu];
                 T tr = T(0), ti = T(0);
                 if constexpr (STAGE) {
-                    const cx<T>* line = buf + ((long long)p * SU + ou + cu) * SV + ov;
+                    const cx<T>* line = buf + ((long long)(p - plo) * SU + ou + cu) * SV + ov;
 #pragma unroll
                     for (int cv = 0; cv < S; ++cv) {
                         const cx<T> v = line[cv];
