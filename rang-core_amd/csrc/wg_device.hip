@@ -787,17 +787,22 @@ __global__ __launch_bounds__(256) void k_zero_split(Geo g, const std::uint32_t* 
     const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
     const int P = g.wgrid ? g.nplanes : 1;
     const int p0 = grp * G, np = min(P, p0 + G) - p0;
-    const int cells = g.Tu * g.Tv;
     const long long plane = (long long)g.nu * g.nv;
-    for (int e = threadIdx.x; e < NOUT * np * cells; e += blockDim.x) {
-        const int o = e / (np * cells), rem0 = e - o * np * cells;
-        const int pl = rem0 / cells, rem = rem0 - pl * cells;
-        const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
-        const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
-        const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
-        grid[(long long)o * P * plane + (long long)(p0 + pl) * plane + (long long)grow * g.nv + gcol] =
-            cx<T>{T(0), T(0)};
-    }
+    // a fixed column per thread, rows in steps (no division per cell)
+    const int istep = int(blockDim.x) / g.Tv;
+    const int i0 = int(threadIdx.x) / g.Tv, j = int(threadIdx.x) - i0 * g.Tv;
+    if (i0 >= istep) return;
+    int gcol = tv * g.Tv + j + g.nv / 2;
+    if (gcol >= g.nv) gcol -= g.nv;
+    for (int o = 0; o < NOUT; ++o)
+        for (int pl = 0; pl < np; ++pl) {
+            cx<T>* base = grid + (long long)o * P * plane + (long long)(p0 + pl) * plane + gcol;
+            for (int i = i0; i < g.Tu; i += istep) {
+                int grow = tu * g.Tu + i + g.nu / 2;
+                if (grow >= g.nu) grow -= g.nu;
+                base[(long long)grow * g.nv] = cx<T>{T(0), T(0)};
+            }
+        }
 }
 
 // The item `item` of a prefix array of per-unit item counts: the unit and
@@ -1126,25 +1131,29 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
     }
     __syncthreads();
     const long long plane = (long long)g.nu * g.nv;
-    const int cells = g.Tu * g.Tv;
-    for (int e = threadIdx.x; e < np * cells; e += NT) {
-        const int pl = e / cells, rem = e - pl * cells;
-        const int i = rem / g.Tv, j = rem - (rem / g.Tv) * g.Tv;
-        const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
-        const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
+    // a fixed column per thread, rows in steps (no division per cell)
+    const int istep = NT / g.Tv;
+    const int i0 = int(threadIdx.x) / g.Tv, j = int(threadIdx.x) - i0 * g.Tv;
+    if (i0 >= istep) return;
+    int gcol = tv * g.Tv + j + g.nv / 2;
+    if (gcol >= g.nv) gcol -= g.nv;
+    for (int pl = 0; pl < np; ++pl)
+        for (int i = i0; i < g.Tu; i += istep) {
+            int grow = tu * g.Tu + i + g.nu / 2;
+            if (grow >= g.nu) grow -= g.nu;
 #pragma unroll
-        for (int o = 0; o < NOUT; ++o) {
-            const GridAcc* are = acc + ((o * np + pl) * 2) * pstride + i * TvP + j;
-            cx<T>* dst = grid + (long long)o * P * plane + (long long)(p0 + pl) * plane +
-                         (long long)grow * g.nv + gcol;
-            if (nchunks == 1) {
-                *dst = cx<T>{T(are[0]), T(are[pstride])};
-            } else {
-                atomicAdd(&dst->re, T(are[0]));
-                atomicAdd(&dst->im, T(are[pstride]));
+            for (int o = 0; o < NOUT; ++o) {
+                const GridAcc* are = acc + ((o * np + pl) * 2) * pstride + i * TvP + j;
+                cx<T>* dst = grid + (long long)o * P * plane + (long long)(p0 + pl) * plane +
+                             (long long)grow * g.nv + gcol;
+                if (nchunks == 1) {
+                    *dst = cx<T>{T(are[0]), T(are[pstride])};
+                } else {
+                    atomicAdd(&dst->re, T(are[0]));
+                    atomicAdd(&dst->im, T(are[pstride]));
+                }
             }
         }
-    }
 }
 
 // ------------------------------------------------------------ transforms
