@@ -1842,14 +1842,25 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
             plo = s_lo;
             phi = min(P, s_hi + S);
         }
-        const int area = SU * SV;
-        for (int e = threadIdx.x; e < (phi - plo) * area; e += blockDim.x) {
-            const int pl = e / area, rem = e - pl * area;
-            const int p = plo + pl;
-            const int i = rem / SV, j = rem - (rem / SV) * SV;
-            const int grow = (tu * g.Tu + i + g.nu / 2) % g.nu;
-            const int gcol = (tv * g.Tv + j + g.nv / 2) % g.nv;
-            buf[e] = grid[(long long)p * plane + (long long)grow * g.nv + vpos(g, gcol)];
+        // a fixed column per thread, rows (plane, i) in steps: no division per cell
+        const int rstep = int(blockDim.x) / SV;
+        const int r0 = int(threadIdx.x) / SV, j = int(threadIdx.x) - r0 * SV;
+        if (r0 < rstep) {
+            int gcol = tv * g.Tv + j + g.nv / 2;
+            if (gcol >= g.nv) gcol -= g.nv;
+            const int col = vpos(g, gcol);
+            const int nrows = (phi - plo) * SU;
+            int pl = r0 / SU, i = r0 - pl * SU;
+            for (int rr = r0; rr < nrows; rr += rstep) {
+                int grow = tu * g.Tu + i + g.nu / 2;
+                if (grow >= g.nu) grow -= g.nu;
+                buf[rr * SV + j] = grid[(long long)(plo + pl) * plane + (long long)grow * g.nv + col];
+                i += rstep;
+                while (i >= SU) {
+                    i -= SU;
+                    ++pl;
+                }
+            }
         }
     }
     __syncthreads();
