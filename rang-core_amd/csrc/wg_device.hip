@@ -1148,7 +1148,8 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                              (long long)grow * g.nv + gcol;
                 if (nchunks == 1) {
                     *dst = cx<T>{T(are[0]), T(are[pstride])};
-                } else {
+                } else if (are[0] != GridAcc(0) || are[pstride] != GridAcc(0)) {
+                    // a chunk adds only the cells its visibilities reached
                     atomicAdd(&dst->re, T(are[0]));
                     atomicAdd(&dst->im, T(are[pstride]));
                 }

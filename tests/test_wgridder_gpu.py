@@ -297,3 +297,20 @@ def test_two_pass_transforms_match_oracle(single, monkeypatch):
                            wgt=wgt.astype(np.float32 if single else np.float64), npix_x=nx, npix_y=ny,
                            pixsize_x=dx, pixsize_y=dx, epsilon=eps, single=single)
         assert rel_linf(d, ref) < TOL_ORACLE[single]
+
+
+def test_cells_no_visibility_reaches_are_cleared():
+    """A call after a denser one: the grid's workspace still holds the first
+    call's planes, and the (tile, plane group)s that none of the second
+    call's visibilities reach are cleared rather than gridded, so nothing of
+    the first call may show in the second's image."""
+    uvw, freq, vis, wgt, nx, ny, dx = ref_case(nx=64, ny=64, nrow=300, seed=21, nchan=4)
+    keep = np.zeros(vis.shape, dtype=np.uint8)
+    keep[:12] = 1
+    for single in (True, False):
+        eps = 1e-5 if single else 1e-10
+        got, ref, _, _ = v2d(uvw, freq, vis, wgt, nx, ny, dx, eps, single)
+        assert rel_linf(got, ref) < TOL_ORACLE[single]
+        got, ref, info, _ = v2d(uvw, freq, vis, wgt, nx, ny, dx, eps, single, mask=keep)
+        assert info["nvis"] == int(keep.sum())
+        assert rel_linf(got, ref) < TOL_ORACLE[single]
