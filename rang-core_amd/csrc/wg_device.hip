@@ -2201,9 +2201,10 @@ template <class T>
 constexpr std::size_t grid_lds() {
     return 160 * 1024 - kGridBatch * sizeof(Rec<T>) - 2 * 10 * 16 * sizeof(T) - 256;
 }
-// Visibilities per gridding item. Larger items were measured slower on the
-// 464M-visibility workload (4096: same, 16384: 1.35x, 65536: 2.6x the time).
-constexpr std::uint32_t kGridChunk = 1024;
+// Visibilities per gridding item, measured on the 464M-visibility workload
+// once a split tile's chunks add only the cells they reach (gridding stage:
+// 512 -> 280.5 ms, 1024 -> 251.6, 2048 -> 238.0, 4096 -> 233.4, 8192 -> 234.5).
+constexpr std::uint32_t kGridChunk = 4096;
 constexpr std::uint32_t kDegridChunk = 1024;     // visibilities per degridding item
 // Staged degridding holds a tile and its halo on every plane in LDS: in
 // single precision one 1024-thread workgroup per CU with up to 150 KiB and
