@@ -5,10 +5,13 @@
 //   plan      kernel, grid and w-planes on the host, FP64    (wg_plan.cpp)
 //   sort      bucket key per visibility, counts, prefix sum, (k_keys, scan,
 //             one record per active visibility in bucket order  k_scatter)
-//   grid      one workgroup per (uv tile, group of w-planes) gathers the
-//             visibilities of its tile and of the three tiles whose
-//             footprints reach into it, accumulates in LDS and writes its
-//             tile of each plane once: no global atomics  (k_grid)
+//   grid      one workgroup per (uv tile, group of w-planes, chunk of up to
+//             4096 visibilities) gathers the visibilities of its tile and of
+//             the three tiles whose footprints reach into it, accumulates in
+//             LDS and writes its tile of each plane once; the chunks of a
+//             split tile add the cells they reached with global atomics, and
+//             cells no visibility reaches are cleared by a fill kernel
+//             (k_grid_items, k_zero_split, k_grid)
 //   v FFT     inverse along v of every row of every plane, keeping the image
 //             columns                                     (k_rows_inv)
 //   u FFT     inverse along u of the image columns, projected with the
