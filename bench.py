@@ -122,6 +122,170 @@ def run_gpu(case, steps, warmup, verbose=False):
     return elapsed, np.asarray(info), means
 
 
+def run_prepared(case, steps, warmup, verbose=False):
+    """vis2dirty + dirty2vis of the workload on a prepared plan (the plan and
+    sorted layouts made once, outside the timed steps; rang-core's
+    wgridder::Prepared): each step one Prepared::vis2dirty of the
+    visibilities and one weighted Prepared::dirty2vis of the resulting image,
+    replayed as graphs after the first two. Returns (elapsed, seconds to
+    prepare, per-stage means of one more step run eagerly with stage timing,
+    the plan, the image and the visibilities of the last step)."""
+    import torch
+
+    from rang import device as D
+    from rang.prepared import Prepared
+
+    uvw, freq, vis, wgt = case["uvw"], case["freq"], case["vis"], case["wgt"]
+    npix, pix, eps = case["npix"], case["pixsize"], case["epsilon"]
+    opts = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = Prepared(opts, uvw, freq, wgt, single=True)
+    torch.cuda.synchronize()
+    t_prepare = time.perf_counter() - t0
+    dirty = torch.empty((npix, npix), dtype=torch.float32, device=vis.device)
+    out_vis = torch.empty_like(vis)
+
+    def step():
+        plan.vis2dirty(vis, out=dirty)
+        plan.dirty2vis(dirty, weighted=True, out=out_vis)
+
+    for _ in range(max(warmup, 2)):  # the first call of each kind eager, the second captured
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        if verbose:
+            progress(f"prepared step {k + 1}/{steps}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    D.stage_timing(True)
+    means = {}
+    plan.vis2dirty(vis, out=dirty)
+    for name, ms in D.stage_times():
+        means[("v2d", name)] = ms
+    plan.dirty2vis(dirty, weighted=True, out=out_vis)
+    for name, ms in D.stage_times():
+        means[("d2v", name)] = ms
+    D.stage_timing(False)
+    torch.cuda.synchronize()
+    return elapsed, t_prepare, means, plan, dirty, out_vis
+
+
+def dft_errors(case, dirty, vis_of, n_pix=256, n_vis=100_000, seed=5):
+    """(L2, Linf/peak) of vis2dirty on `n_pix` random pixels (the brightest
+    source's pixel first) against the FP64 direct sum over every visibility,
+    and of dirty2vis of a 20-point image (`vis_of(image)` the gridder's
+    visibilities) on `n_vis` random visibilities against its direct sum."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    from direct import dirty_pixels, errors, vis_of_points
+
+    n, pix = case["npix"], case["pixsize"]
+    dev = case["vis"].device
+    g = torch.Generator(device=dev).manual_seed(seed)
+    i = torch.randint(0, n, (n_pix,), device=dev, generator=g)
+    j = torch.randint(0, n, (n_pix,), device=dev, generator=g)
+    flux, sl, sm = case["sky"]
+    b = int(np.argmax(flux))
+    i[0], j[0] = int(round(n // 2 + sl[b] / pix)), int(round(n // 2 + sm[b] / pix))
+    ref = dirty_pixels(case["uvw"], case["freq"], case["vis"], case["wgt"], i, j, n, n, pix, pix)
+    e_v2d = errors(dirty[i, j].double().cpu().numpy(), ref.cpu().numpy())
+    rng = np.random.default_rng(seed)
+    pi, pj = rng.integers(n // 4, 3 * n // 4, 20), rng.integers(n // 4, 3 * n // 4, 20)
+    pf = rng.uniform(0.1, 1.0, 20)
+    img = torch.zeros((n, n), dtype=dirty.dtype, device=dev)
+    img[torch.from_numpy(pi).to(dev), torch.from_numpy(pj).to(dev)] = torch.from_numpy(pf).to(dev).to(dirty.dtype)
+    vis = vis_of(img)
+    idx = torch.randint(0, vis.numel(), (n_vis,), device=dev, generator=g)
+    nchan = case["freq"].numel()
+    refv = vis_of_points(case["uvw"], case["freq"], idx // nchan, idx % nchan, torch.from_numpy(pi).to(dev),
+                         torch.from_numpy(pj).to(dev), torch.from_numpy(pf).to(dev), n, n, pix, pix)
+    refv = refv * case["wgt"].reshape(-1)[idx].double()
+    e_d2v = errors(vis.reshape(-1)[idx].to(torch.complex128).cpu().numpy(), refv.cpu().numpy())
+    return {
+        "vis2dirty_vs_fp64_dft": {"l2": e_v2d[0], "linf_over_peak": e_v2d[1], "sample": f"{n_pix} pixels"},
+        "dirty2vis_vs_fp64_dft": {"l2": e_d2v[0], "linf_over_peak": e_d2v[1],
+                                  "sample": f"{n_vis} visibilities of a 20-point image"},
+    }
+
+
+def run_f64(case, steps):
+    """The scaled-up workload in double precision: one-shot vis2dirty +
+    dirty2vis calls (complex128 visibilities, float64 weights), timed over
+    `steps` after one warm-up, with the direct-transform errors."""
+    import torch
+
+    from rang import device as D
+
+    npix, pix, eps = case["npix"], case["pixsize"], case["epsilon"]
+    opts = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    c64 = dict(case)
+    c64["vis"] = case["vis"].to(torch.complex128)
+    c64["wgt"] = case["wgt"].to(torch.float64)
+    dirty = torch.empty((npix, npix), dtype=torch.float64, device=c64["vis"].device)
+    out_vis = torch.empty_like(c64["vis"])
+
+    def step():
+        _, _, info = D.vis2dirty(opts, c64["uvw"], c64["freq"], c64["vis"], c64["wgt"], single=False, out=(dirty, None))
+        D.dirty2vis(opts, c64["uvw"], c64["freq"], dirty, c64["wgt"], out=out_vis)
+        return info
+
+    info = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        info = step()
+        progress(f"f64 step {k + 1}/{steps}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    errs = dft_errors(c64, dirty, lambda img: D.dirty2vis(opts, c64["uvw"], c64["freq"], img, c64["wgt"])[0])
+    nvis = c64["vis"].numel()
+    del c64, out_vis
+    torch.cuda.empty_cache()
+    return {
+        "value": round(nvis * steps / elapsed / 1e6, 2),
+        "unit": "Mvis/s",
+        "ms_per_step": round(elapsed / steps * 1e3, 1),
+        "grid": [int(info[2]), int(info[3])],
+        "support": int(info[1]),
+        "w_planes": int(info[4]),
+        "calls": "one-shot vis2dirty + dirty2vis (scan, plan, sort, grid, transforms each call)",
+        "errors": errs,
+    }
+
+
+def run_host_arrays(case, steps=1):
+    """vis2dirty + dirty2vis through the host-array entry points of rang._core
+    (the reference's boundary): the arrays cross PCIe both ways inside the
+    timed region. Returns Mvis/s and seconds per step."""
+    import numpy as np
+
+    from rang import _core
+
+    npix, pix, eps = case["npix"], case["pixsize"], case["epsilon"]
+    opts = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    uvw, freq = case["uvw"].cpu().numpy(), case["freq"].cpu().numpy()
+    vis, wgt = case["vis"].cpu().numpy(), case["wgt"].cpu().numpy()
+    dirty = np.zeros((npix, npix), np.float32)
+    out = np.zeros(vis.shape, np.complex64)
+    info = np.zeros(_core.N_INFO)
+    times = []
+    for _ in range(steps + 1):
+        t0 = time.perf_counter()
+        st = _core.wgridder_vis2dirty_f32(opts, uvw, freq, vis, wgt, None, None, dirty, None, info)
+        st |= _core.wgridder_dirty2vis_f32(opts, uvw, freq, dirty, wgt, None, out.view(np.float32), info)
+        times.append(time.perf_counter() - t0)
+        if st:
+            raise RuntimeError(_core.wgridder_last_error())
+    t = statistics.fmean(times[1:])
+    return {"value": round(vis.size / t / 1e6, 2), "unit": "Mvis/s", "seconds_per_step": round(t, 3),
+            "boundary": "rang_core_wgridder_vis2dirty_ex_f32 + dirty2vis_ex_f32 on host arrays (PCIe both ways)"}
+
+
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE scaled by a plain
@@ -175,8 +339,13 @@ def _threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
-def cpu_small(case, repeats):
-    """The oracle on the whole small workload (median of `repeats`)."""
+def cpu_small(case, repeats, gpu_dirty=None, gpu_vis=None):
+    """The oracle on the whole small workload (median of `repeats`); with the
+    GPU's image and visibilities (the dirty2vis of that image), also their
+    errors against the oracle's: vis2dirty of the same data, dirty2vis of the
+    same image. Returns (baseline, errors or None)."""
+    import numpy as np
+
     from oracle import oracle as O
 
     npix, pix, eps = case["npix"], case["pixsize"], case["epsilon"]
@@ -189,6 +358,17 @@ def cpu_small(case, repeats):
                     pixsize_y=pix, epsilon=eps)
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
+    errs = None
+    if gpu_dirty is not None:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from direct import errors
+
+        e1 = errors(gpu_dirty, imgs["dirty"])
+        vref, _ = O.dirty2vis(case["uvw"], case["freq"], np.ascontiguousarray(gpu_dirty), case["wgt"],
+                              pixsize_x=pix, pixsize_y=pix, epsilon=eps)
+        e2 = errors(gpu_vis, vref)
+        errs = {"vis2dirty_vs_oracle": {"l2": e1[0], "linf_over_peak": e1[1], "sample": "every pixel"},
+                "dirty2vis_vs_oracle": {"l2": e2[0], "linf_over_peak": e2[1], "sample": "every visibility"}}
     return {
         "value": round(case["vis"].size / t / 1e6, 3),
         "unit": "Mvis/s",
@@ -196,7 +376,7 @@ def cpu_small(case, repeats):
         "kind": "port",
         "sample": f"the whole small workload ({case['vis'].size} visibilities), vis2dirty + "
                   f"dirty2vis, median of {repeats} runs of the oracle on {_threads()} threads",
-    }
+    }, errs
 
 
 def cpu_large(info, nvis_total):
@@ -436,15 +616,33 @@ def main():
     from rang import synth
 
     dev = torch.device("cuda:0")
+    from rang import device as D
+
     if not args.only_small:
         big = synth.config2_device(dev)
         progress("visibilities formed")
         nrow, nchan = int(big["uvw"].shape[0]), int(big["freq"].shape[0])
         nvis_total = nrow * nchan
-        elapsed, info, means = run_gpu(big, args.steps, args.warmup, verbose=True)
+        # the plan cached (the metric's definition): prepared plan, graph replays
+        elapsed, t_prepare, means, plan, dirty, out_vis = run_prepared(big, args.steps, args.warmup, verbose=True)
+        _, _, info = D.vis2dirty([big["npix"], big["npix"], big["pixsize"], big["pixsize"], big["epsilon"], 0, 0,
+                                  1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0], big["uvw"], big["freq"], big["vis"],
+                                 big["wgt"], single=True, out=(dirty, None))
         value, roofline, per_call = summarise(big, nvis_total, nrow, nchan, args.steps, elapsed,
                                               info, means, workload="large")
-        progress(f"large workload: {value:.1f} Mvis/s")
+        progress(f"large workload, prepared: {value:.1f} Mvis/s")
+        errs_large = dft_errors(big, dirty, lambda img: plan.dirty2vis(img, weighted=True))
+        plan.close()
+        del plan, out_vis
+        torch.cuda.empty_cache()
+        # every call in full (scan, plan, sort each time)
+        f_el, f_info, f_means = run_gpu(big, max(2, args.steps // 2), 1)
+        full_value = nvis_total * max(2, args.steps // 2) / f_el / 1e6
+        progress(f"large workload, full calls: {full_value:.1f} Mvis/s")
+        host_large = run_host_arrays(big, 1)
+        progress(f"large workload, host arrays: {host_large['value']} Mvis/s")
+        f64 = run_f64(big, 2)
+        progress(f"large workload, f64: {f64['value']} Mvis/s")
         del big
         torch.cuda.empty_cache()
 
@@ -454,21 +652,38 @@ def main():
         dcase = dict(small)
         for k in ("uvw", "freq", "vis", "wgt"):
             dcase[k] = torch.from_numpy(small[k]).to(dev)
-        s_el, s_info, s_means = run_gpu(dcase, 20, 3)
+        s_el, s_tp, s_means, s_plan, s_dirty, s_vis = run_prepared(dcase, 50, 3)
+        _, _, s_info = D.vis2dirty([1024, 1024, small["pixsize"], small["pixsize"], 1e-5, 0, 0, 1.1, 2.6, 1, 0, 0, 0,
+                                    0, 1, -1, 0, 0], dcase["uvw"], dcase["freq"], dcase["vis"], dcase["wgt"],
+                                   single=True)
         s_value, s_roof, s_call = summarise(dcase, small["vis"].size, small["uvw"].shape[0],
-                                            small["freq"].size, 20, s_el, s_info, s_means)
+                                            small["freq"].size, 50, s_el, s_info, s_means)
+        s_errs = dft_errors(dcase, s_dirty, lambda img: s_plan.dirty2vis(img, weighted=True))
+        sf_el, _, _ = run_gpu(dcase, 20, 3)
+        s_host = run_host_arrays(dcase, 3)
         small_line = {
             "workload": "2016 baselines x 100 times x 16 channels (3,225,600 vis), 1024x1024 at "
                         "2 arcsec, eps 1e-5, w-gridding, single precision",
             "value": round(s_value, 2),
-            "ms_per_step": round(s_el / 20 * 1e3, 4),
+            "ms_per_step": round(s_el / 50 * 1e3, 4),
+            "prepare_seconds": round(s_tp, 4),
+            "full_calls": {"value": round(small["vis"].size * 20 / sf_el / 1e6, 2),
+                           "ms_per_step": round(sf_el / 20 * 1e3, 4)},
+            "host_arrays": s_host,
             "grid": [int(s_info[2]), int(s_info[3])],
             "w_planes": int(s_info[4]),
             "roofline": s_roof,
             "per_call": s_call,
             "stages_ms": {f"{d}:{n}": round(v, 4) for (d, n), v in sorted(s_means.items())},
-            "cpu_baseline": None if args.no_cpu_baseline else cpu_small(small, args.cpu_repeats),
+            "errors": s_errs,
+            "cpu_baseline": None,
         }
+        if not args.no_cpu_baseline:
+            base, oerr = cpu_small(small, args.cpu_repeats, s_dirty.cpu().numpy(),
+                                   s_plan.dirty2vis(s_dirty, weighted=True).cpu().numpy())
+            small_line["cpu_baseline"] = base
+            small_line["errors"].update(oerr)
+        s_plan.close()
         progress(f"small workload: {s_value:.1f} Mvis/s")
         if args.only_small:
             print(json.dumps({"config_small": small_line}))
@@ -478,7 +693,7 @@ def main():
     gram = None
     if not args.no_gram:
         gram = gram_solve(dev)
-        progress(f"Gram solve: CG {gram['cg_applies']} applies, learned {gram['learned_applies']}")
+        progress(f"Gram solve: {json.dumps({k: gram[k] for k in gram if k.endswith('applies')})}")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -495,7 +710,7 @@ def main():
         "config": {
             "workload": "wgridder vis2dirty+dirty2vis, 2016 baselines x 900 times x 256 channels "
                         "(464,486,400 vis, 0.9-1.7 GHz), 8192x8192 at 1 arcsec, eps 1e-5, "
-                        "w-gridding, single precision",
+                        "w-gridding, single precision, plan cached (prepared once, outside the timed steps)",
             "nvis": nvis_total,
             "image": [8192, 8192],
             "grid": [int(info[2]), int(info[3])],
@@ -503,6 +718,12 @@ def main():
             "w_planes": int(info[4]),
             "parallelism": "one GPU",
         },
+        "prepare_seconds": round(t_prepare, 3),
+        "full_calls": {"value": round(full_value, 2), "ms_per_step": round(f_el / max(2, args.steps // 2) * 1e3, 3),
+                       "stages_ms": {f"{d}:{n}": round(v, 3) for (d, n), v in sorted(f_means.items())}},
+        "host_arrays": host_large,
+        "f64": f64,
+        "errors": errs_large,
         "stages_ms": {f"{d}:{n}": round(v, 3) for (d, n), v in sorted(means.items())},
         "roofline": roofline,
         "per_call": per_call,

@@ -39,7 +39,10 @@ typedef struct {
 #define RANG_GRID_N_TIMINGS 14
 
 /* core/src/ffi_grid.rs:54-145: invert to a dirty image and PSF; 0 ok, 1
- * invalid arguments, 2 a failure inside. Runs the double-precision GPU path. */
+ * invalid arguments, 2 a failure inside. Runs the double-precision GPU path,
+ * with transforms of any padded side as the reference's rustfft takes them
+ * (sides with prime factors above 11 by Bluestein's chirp transform, sides
+ * beyond one LDS in four steps). */
 int32_t rang_core_grid_invert(const GridInputs *inputs, GridOutputs *outputs);
 
 /* core/src/ffi_grid.rs:154-170: the oversampled Kaiser-Bessel kernel,
@@ -50,7 +53,8 @@ int32_t rang_core_grid_kernel(size_t kernel_size, size_t oversampling, double *k
  * doubles (context, compile, prepare, bucket, upload, compute, grid, fft,
  * project, download, finish seconds; device bytes, batches, layers per
  * batch); 0 ok, 1 invalid arguments, 2 a failure inside, 3 the GPU path
- * failed. */
+ * failed. As the reference's GPU path, padded sides with a prime factor
+ * above 11 are refused (3). */
 int32_t rang_core_grid_invert_gpu(const GridInputs *inputs, GridOutputs *outputs, int32_t precision,
                                   double *timings);
 

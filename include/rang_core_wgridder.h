@@ -212,13 +212,32 @@ int32_t rang_amd_wgridder_prepared_dirty2vis_f32(RangPrepared *p, const float *d
                                                  int32_t weighted, float *vis, void *stream);
 /* Prepared::residual (:3230-3255, 3300-3383): the image of vis - A model (model
  * NULL: of vis) with the prepared weights, and misfit = sum w |vis - A model|^2
- * (a parallel sum in double, not the reference's serial order). */
+ * (a parallel sum in double, not the reference's serial order). The
+ * prediction and residual are formed in the degridder and handed to the
+ * gridder in one value per active visibility; no visibility-sized buffer. */
 int32_t rang_amd_wgridder_prepared_residual_f64(RangPrepared *p, const void *vis,
                                                 int32_t vis_kind, const double *model,
                                                 double *dirty, double *misfit, void *stream);
 int32_t rang_amd_wgridder_prepared_residual_f32(RangPrepared *p, const void *vis,
                                                 int32_t vis_kind, const float *model,
                                                 float *dirty, double *misfit, void *stream);
+/* Prepared::residual_sorted (:3257-3267): as _residual with the data held only
+ * for the active visibilities, in the order _sorted gives (the prepared
+ * plan's gridding order); no visibility-sized array is read or held. */
+int32_t rang_amd_wgridder_prepared_residual_sorted_f64(RangPrepared *p, const void *vis,
+                                                       int32_t vis_kind, const double *model,
+                                                       double *dirty, double *misfit,
+                                                       void *stream);
+int32_t rang_amd_wgridder_prepared_residual_sorted_f32(RangPrepared *p, const void *vis,
+                                                       int32_t vis_kind, const float *model,
+                                                       float *dirty, double *misfit, void *stream);
+/* Prepared::sorted (:3269-3286): `dense` (nrow x nchan elements of elem_bytes
+ * = 4, 8 or 16, device memory) at the active visibilities, in that order,
+ * into `out` (active elements). */
+int32_t rang_amd_wgridder_prepared_sorted_f64(RangPrepared *p, const void *dense, size_t elem_bytes,
+                                              void *out, void *stream);
+int32_t rang_amd_wgridder_prepared_sorted_f32(RangPrepared *p, const void *dense, size_t elem_bytes,
+                                              void *out, void *stream);
 /* The normal operator A^H W A x of the prepared plan (WgridMeasurement::
  * gram_apply, core/src/clean_certified.rs:233-236): out = vis2dirty(dirty2vis(x)). */
 int32_t rang_amd_wgridder_prepared_gram_f64(RangPrepared *p, const double *x, double *out,

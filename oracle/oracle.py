@@ -366,3 +366,70 @@ def ws_invert(uvw, frequencies, data, weights, imaging_weights, *, width, height
         raise ValueError("invalid w-stacking inputs")
     rec = dict(zip(WS_STAT_NAMES, stats.tolist()))
     return {"dirty": dirty, "psf": psf, "stats": rec}
+
+
+# --------------------------------------------------------------- weights
+# The serial restatement of core/src/weights.rs (oracle/weights.c).
+
+WEIGHT_CLASSES = {"natural": 0, "uniform": 1, "briggs": 2}
+
+
+def _weights_lib():
+    L = lib()
+    if not getattr(L, "_weights_bound", False):
+        vp, sz, i32, dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+        L.or_padded_size.argtypes = [sz, dp]
+        L.or_padded_size.restype = sz
+        L.or_grid_shape.argtypes = [sz, sz, dp, vp]
+        L.or_grid_shape.restype = i32
+        L.or_stokes_i.argtypes = [sz, vp, vp, vp, vp, vp]
+        L.or_stokes_i.restype = i32
+        L.or_imaging_weights.argtypes = [sz, sz, vp, vp, vp, vp, i32, dp, sz, sz, dp, dp, dp, dp, sz, vp, vp, vp]
+        L.or_imaging_weights.restype = i32
+        L._weights_bound = True
+    return L
+
+
+def padded_size(trimmed, padding=1.2):
+    return int(_weights_lib().or_padded_size(int(trimmed), float(padding)))
+
+
+def grid_shape(width, height, super_weight=1.0):
+    out = np.zeros(2, np.uintp)
+    if _weights_lib().or_grid_shape(int(width), int(height), float(super_weight), _p(out)):
+        raise ValueError("positive super weight required")
+    return int(out[0]), int(out[1])
+
+
+def stokes_i(data, weights, flags):
+    data = np.ascontiguousarray(data, np.complex64)
+    weights = np.ascontiguousarray(weights, np.float32)
+    flags = np.ascontiguousarray(flags, np.uint8)
+    shape = data.shape[:-1]
+    n = int(np.prod(shape))
+    vis = np.zeros(shape, np.complex64)
+    w = np.zeros(shape, np.float32)
+    _weights_lib().or_stokes_i(n, _p(data), _p(weights), _p(flags), _p(vis), _p(w))
+    return vis, w
+
+
+def imaging_weights(uvw, wavelength, weights, selected, *, mode="briggs", robust=0.5, width=616, height=616,
+                    pixel_scale=8.0 * np.pi / (180.0 * 3600.0), super_weight=1.0, rank_filter=(3.0, 16)):
+    """(weights f32 (nrow, nchan), grid (h/2, w), sums [vis weight sum, total, max, count])."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    wavelength = np.ascontiguousarray(wavelength, np.float64)
+    weights = np.ascontiguousarray(weights, np.float32)
+    selected = np.ascontiguousarray(selected, np.uint8)
+    nrow, nchan = weights.shape
+    gw, gh = grid_shape(width, height, super_weight)
+    out = np.zeros((nrow, nchan), np.float32)
+    grid = np.zeros((gh // 2, gw))
+    sums = np.zeros(4)
+    level, size = rank_filter if rank_filter is not None else (0.0, 0)
+    st = _weights_lib().or_imaging_weights(nrow, nchan, _p(uvw), _p(wavelength), _p(weights), _p(selected),
+                                           WEIGHT_CLASSES[mode], float(robust), int(width), int(height),
+                                           float(pixel_scale), float(pixel_scale), float(super_weight),
+                                           float(level), int(size), _p(out), _p(grid), _p(sums))
+    if st:
+        raise ValueError("invalid imaging-weight arguments")
+    return out, grid, sums

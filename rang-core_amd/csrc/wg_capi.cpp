@@ -5,13 +5,17 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
+#include <stdexcept>
 #include <vector>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 
+#include "../../include/rang_core_gram.h"
 #include "../../include/rang_core_wgridder.h"
+#include "gram.h"
 #include "wg_device.h"
 #include "wg_plan.h"
 
@@ -239,10 +243,15 @@ int32_t d2v_host(const WgridderOptions* opts, std::size_t nrow, std::size_t ncha
     }
 }
 
+// The channel frequencies from the device, read in order on the call's
+// stream (a write to them queued on that stream is seen).
 bool freq_to_host(const double* freq_dev, std::size_t nchan, std::vector<double>& out,
-                  std::string& err) {
+                  void* stream, std::string& err) {
     out.assign(nchan, 0.0);
-    if (nchan && hipMemcpy(out.data(), freq_dev, sizeof(double) * nchan, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (!nchan) return true;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(out.data(), freq_dev, sizeof(double) * nchan, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
         err = "cannot read the channel frequencies from the device";
         return false;
     }
@@ -260,7 +269,7 @@ int32_t v2d_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan
         std::string err;
         if (!kinds_ok(vis, vis_kind, wgt, wgt2, wgt_kind, err)) return fail(1, err);
         std::vector<double> fh;
-        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err);
+        if (!freq_to_host(freq, nchan, fh, stream, err)) return fail(2, err);
         DevInputs in;
         in.nrow = nrow;
         in.nchan = nchan;
@@ -291,7 +300,7 @@ int32_t d2v_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan
         if (wgt_kind == kWgtF32Product || !kinds_ok(nullptr, kVisUnit, wgt, nullptr, wgt_kind, err))
             return fail(1, err.empty() ? "invalid weight array or kind" : err);
         std::vector<double> fh;
-        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err);
+        if (!freq_to_host(freq, nchan, fh, stream, err)) return fail(2, err);
         DevInputs in;
         in.nrow = nrow;
         in.nchan = nchan;
@@ -357,7 +366,7 @@ RangPrepared* prepare_dev(const WgridderOptions* opts, std::size_t nrow, std::si
         std::string err;
         if (!kinds_ok(nullptr, kVisUnit, wgt, nullptr, wgt_kind, err)) return fail(1, err), nullptr;
         std::vector<double> fh;
-        if (!freq_to_host(freq, nchan, fh, err)) return fail(2, err), nullptr;
+        if (!freq_to_host(freq, nchan, fh, stream, err)) return fail(2, err), nullptr;
         DevInputs in;
         in.nrow = nrow;
         in.nchan = nchan;
@@ -590,6 +599,27 @@ int32_t rang_amd_wgridder_dirty2vis_dev_f32(const WgridderOptions* opts, size_t 
                                         static_cast<hipStream_t>(stream), err);                    \
         });                                                                                         \
     }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_residual_sorted_##SUF(RangPrepared* h, const void* vis,     \
+                                                             int32_t vis_kind, const T* model,     \
+                                                             T* dirty, double* misfit,             \
+                                                             void* stream) {                       \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!vis_ok(vis, vis_kind, err)) return 1;                                             \
+            if (!dirty || !misfit) return err = "null array", 1;                                   \
+            return prepared_residual_sorted<T>(p, vis, vis_kind, model, dirty, misfit,             \
+                                               static_cast<hipStream_t>(stream), err);             \
+        });                                                                                         \
+    }                                                                                               \
+    int32_t rang_amd_wgridder_prepared_sorted_##SUF(RangPrepared* h, const void* dense,            \
+                                                    size_t elem_bytes, void* out, void* stream) {  \
+        return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
+            if (!dense || !out) return err = "null array", 1;                                      \
+            if (elem_bytes != 4 && elem_bytes != 8 && elem_bytes != 16)                             \
+                return err = "elements of 4, 8 or 16 bytes", 1;                                     \
+            return prepared_sorted<T>(p, dense, int(elem_bytes), out,                              \
+                                      static_cast<hipStream_t>(stream), err);                      \
+        });                                                                                         \
+    }                                                                                               \
     int32_t rang_amd_wgridder_prepared_gram_##SUF(RangPrepared* h, const T* x, T* out,             \
                                                   void* stream) {                                  \
         return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {                        \
@@ -656,6 +686,151 @@ RANG_PREPARED_ABI(f32, float)
 RANG_PLAN_ABI(f64, double)
 RANG_PLAN_ABI(f32, float)
 #undef RANG_PLAN_ABI
+
+// ------------------------------------------------------------ Gram operator
+
+int32_t rang_amd_wexpansion(int32_t stencil, double theta, double ridge, double* sup_error, double* lebesgue,
+                            double* table) {
+    try {
+        if (!sup_error || !lebesgue) return fail(1, "null array");
+        WExpansion ex;
+        std::string err;
+        if (!wexpansion_build(stencil, theta, ridge, ex, err)) return fail(1, err);
+        *sup_error = ex.sup_error;
+        *lebesgue = ex.lebesgue;
+        if (table) std::memcpy(table, ex.table.data(), sizeof(double) * ex.table.size());
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the Gram operator: ") + e.what());
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+RangGram* gram_make(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan, const double* uvw,
+                    const double* freq, const void* wgt, int32_t wgt_kind, bool with_w, int32_t stencil,
+                    double theta, double ridge, double epsilon, void* stream) {
+    try {
+        if (!opts) return fail(1, "null options"), nullptr;
+        if (!uvw || !freq || !wgt) return fail(1, "null array"), nullptr;
+        if (wgt_kind != kWgtF32 && wgt_kind != kWgtF64) return fail(1, "invalid weight array or kind"), nullptr;
+        std::string err;
+        WExpansion ex;
+        if (with_w && !wexpansion_build(stencil, theta, ridge, ex, err)) return fail(1, err), nullptr;
+        std::vector<double> fh;
+        if (!freq_to_host(freq, nchan, fh, stream, err)) return fail(2, err), nullptr;
+        DevInputs in;
+        in.nrow = nrow;
+        in.nchan = nchan;
+        in.uvw = uvw;
+        in.freq = freq;
+        in.wgt = wgt;
+        in.wgt_kind = wgt_kind;
+        int status = 0;
+        GramBase* g = gram_new<T>(to_options(*opts), in, fh.data(), with_w ? &ex : nullptr, epsilon,
+                                  static_cast<hipStream_t>(stream), status, err);
+        if (!g) fail(status ? status : 2, err);
+        return reinterpret_cast<RangGram*>(g);
+    } catch (const std::exception& e) {
+        fail(2, std::string("failure in the Gram operator: ") + e.what());
+        return nullptr;
+    }
+}
+
+template <class T>
+int32_t gram_apply_abi(RangGram* h, const T* x, T* out, void* stream) {
+    try {
+        GramBase* g = reinterpret_cast<GramBase*>(h);
+        if (!g || !x || !out) return fail(1, "null array");
+        if (g->single() != (sizeof(T) == 4)) return fail(1, "Gram operator of the other precision");
+        std::string err;
+        const int st = gram_apply_dev<T>(g, x, out, static_cast<hipStream_t>(stream), err);
+        return st ? fail(st, err) : 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the Gram operator: ") + e.what());
+    }
+}
+
+template <class T>
+int32_t pcg_abi(RangPrepared* h, RangGram* m, const T* rhs, T* x, int32_t iterations, double tolerance,
+                int32_t* used, double* rel, void* stream) {
+    return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {
+        GramBase* g = reinterpret_cast<GramBase*>(m);
+        if (!rhs || !x || !used || !rel || !g) return err = "null array", 1;
+        if (g->single() != (sizeof(T) == 4)) return err = "Gram operator of the other precision", 1;
+        int u = 0;
+        std::function<void(const T*, T*, hipStream_t)> pre = [g](const T* r, T* z, hipStream_t s) {
+            std::string e;
+            if (gram_apply_dev<T>(g, r, z, s, e)) throw std::runtime_error(e);
+        };
+        const int st = prepared_pcg<T>(p, pre, rhs, x, iterations, tolerance, &u, rel,
+                                       static_cast<hipStream_t>(stream), err);
+        *used = u;
+        return st;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+#define RANG_GRAM_ABI(SUF, T)                                                                          \
+    RangGram* rang_amd_gram_new_##SUF(const WgridderOptions* opts, size_t nrow, size_t nchan,          \
+                                      const double* uvw, const double* freq, const void* wgt,          \
+                                      int32_t wgt_kind, int32_t stencil, double theta, double ridge,   \
+                                      double epsilon, void* stream) {                                  \
+        return gram_make<T>(opts, nrow, nchan, uvw, freq, wgt, wgt_kind, true, stencil, theta, ridge,   \
+                            epsilon, stream);                                                          \
+    }                                                                                                   \
+    RangGram* rang_amd_gram_no_w_##SUF(const WgridderOptions* opts, size_t nrow, size_t nchan,         \
+                                       const double* uvw, const double* freq, const void* wgt,         \
+                                       int32_t wgt_kind, double epsilon, void* stream) {               \
+        return gram_make<T>(opts, nrow, nchan, uvw, freq, wgt, wgt_kind, false, 0, 0.0, 0.0, epsilon,  \
+                            stream);                                                                   \
+    }                                                                                                   \
+    int32_t rang_amd_gram_apply_##SUF(RangGram* g, const T* x, T* out, void* stream) {                 \
+        return gram_apply_abi<T>(g, x, out, stream);                                                    \
+    }                                                                                                   \
+    int32_t rang_amd_wgridder_prepared_pcg_##SUF(RangPrepared* p, RangGram* m, const T* rhs, T* x,     \
+                                                 int32_t iterations, double tolerance, int32_t* used,  \
+                                                 double* rel, void* stream) {                          \
+        return pcg_abi<T>(p, m, rhs, x, iterations, tolerance, used, rel, stream);                      \
+    }
+RANG_GRAM_ABI(f64, double)
+RANG_GRAM_ABI(f32, float)
+#undef RANG_GRAM_ABI
+
+RangGram* rang_amd_gram_inverse_filter(const RangGram* h, double rho, void* stream) {
+    try {
+        const GramBase* g = reinterpret_cast<const GramBase*>(h);
+        if (!g) return fail(1, "null Gram operator"), nullptr;
+        int status = 0;
+        std::string err;
+        GramBase* out = gram_inverse_filter(g, rho, static_cast<hipStream_t>(stream), status, err);
+        if (!out) fail(status ? status : 2, err);
+        return reinterpret_cast<RangGram*>(out);
+    } catch (const std::exception& e) {
+        fail(2, std::string("failure in the Gram operator: ") + e.what());
+        return nullptr;
+    }
+}
+
+int32_t rang_amd_gram_stats(const RangGram* h, double* out) {
+    const GramBase* g = reinterpret_cast<const GramBase*>(h);
+    if (!g || !out) return fail(1, "null array");
+    g->stats(out);
+    return 0;
+}
+
+size_t rang_amd_gram_side(const RangGram* h) {
+    const GramBase* g = reinterpret_cast<const GramBase*>(h);
+    return g ? size_t(g->side()) : 0;
+}
+
+void rang_amd_gram_free(RangGram* h) { delete reinterpret_cast<GramBase*>(h); }
 
 size_t rang_amd_wgridder_prepared_active(const RangPrepared* h) {
     const PreparedBase* p = reinterpret_cast<const PreparedBase*>(h);

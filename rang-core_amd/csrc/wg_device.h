@@ -70,6 +70,15 @@ int prepared_dirty2vis(PreparedBase* p, const T* dirty, bool weighted, T* vis, h
 template <class T>
 int prepared_residual(PreparedBase* p, const void* vis, int vis_kind, const T* model, T* dirty,
                       double* misfit, hipStream_t stream, std::string& err);
+// Prepared::residual_sorted: the data in the order of prepared_sorted (one
+// value per active visibility).
+template <class T>
+int prepared_residual_sorted(PreparedBase* p, const void* vis, int vis_kind, const T* model, T* dirty,
+                             double* misfit, hipStream_t stream, std::string& err);
+// Prepared::sorted: out = a dense nrow x nchan array (elements of `bytes`:
+// 4, 8 or 16) at the active visibilities in the residual's order.
+template <class T>
+int prepared_sorted(PreparedBase* p, const void* dense, int bytes, void* out, hipStream_t stream, std::string& err);
 template <class T>
 int prepared_gram(PreparedBase* p, const T* x, T* out, hipStream_t stream, std::string& err);
 template <class T>

@@ -38,6 +38,7 @@
 #include <utility>
 #include <vector>
 
+#include "gram.h"
 #include "wg_device.h"
 #include "wg_fft.h"
 
@@ -47,6 +48,7 @@ namespace {
 
 struct DevError {
     std::string msg;
+    int status = 2;  // 2: failure on the device; 1: a request beyond a fixed limit
 };
 
 void check(hipError_t e, const char* what) {
@@ -61,6 +63,57 @@ void check(hipError_t e, const char* what) {
 unsigned blocks_of(long long blocks, int threads) {
     if (blocks <= 0 || blocks * threads >= (1LL << 32)) throw DevError{"launch above 2^32 work-items"};
     return unsigned(blocks);
+}
+
+// Zero fill by a kernel rather than hipMemsetAsync, so a stream that is
+// captured into a graph records a kernel node: 16-byte stores over the body
+// when the buffer is 16-byte aligned, 4-byte stores otherwise and for the tail.
+__global__ __launch_bounds__(256) void k_fill0(std::uint32_t* p, long long words, int vec) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long quads = vec ? words / 4 : 0;
+    uint4* q = reinterpret_cast<uint4*>(p);
+    for (long long k = i; k < quads; k += stride) q[k] = make_uint4(0u, 0u, 0u, 0u);
+    for (long long k = quads * 4 + i; k < words; k += stride) p[k] = 0u;
+}
+
+// Raises a kernel's dynamic LDS limit once per size: later launches (and
+// launches under graph capture) make no attribute call.
+void lds_limit(const void* kernel, std::size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, std::size_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(kernel);
+    if (it != done.end() && it->second >= bytes) return;
+    check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)),
+          "hipFuncSetAttribute");
+    done[kernel] = bytes;
+}
+
+// total = sum of n partial sums in a fixed order (one workgroup of 256).
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, long long n,
+                                                   double* __restrict__ total) {
+    __shared__ double s[256];
+    double v = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) v += part[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = s[0];
+}
+
+void fill_zero(void* p, std::size_t bytes, hipStream_t st) {
+    if (bytes == 0) return;
+    if (bytes % 4 || reinterpret_cast<std::uintptr_t>(p) % 4) throw DevError{"zero fill of an unaligned buffer"};
+    const long long words = (long long)(bytes / 4);
+    const int vec = reinterpret_cast<std::uintptr_t>(p) % 16 == 0 ? 1 : 0;
+    const long long work = vec ? (words + 3) / 4 : words;
+    const long long blocks = std::max<long long>(1, std::min<long long>((work + 255) / 256, 8192));
+    k_fill0<<<unsigned(blocks), 256, 0, st>>>(static_cast<std::uint32_t*>(p), words, vec);
+    check(hipGetLastError(), "k_fill0");
 }
 
 // ------------------------------------------------------------ workspace
@@ -103,6 +156,9 @@ enum Slot : int {
     kSlotSolve,
     kSlotLong,  // twiddles of the transforms longer than one LDS
     kSlotFista,  // the image vectors of the FISTA solve
+    kSlotScalars,  // device scalars of a call (sums)
+    kSlotItems2,   // degridding items of a prepared plan (gridding keeps kSlotItems)
+    kSlotItemOff2,
     kSlotCount
 };
 
@@ -115,12 +171,14 @@ struct Workspace {
     // Pinned host staging for the small transfers of a call.
     void* pinned = nullptr;
     std::size_t pinned_size = 0;
+    unsigned gen = 0;  // counts reallocations (a graph holds the old addresses)
     // Signature of the plan whose tables (twiddles, coefficients, screen and
     // correction tables, tile order) the table slots hold, and their layout.
     std::vector<double> table_key;
     FftDesc du, dv;
     std::vector<double> foc_key;
     std::vector<int> order_key[2];
+    std::vector<double> long_key;
     void* host(std::size_t bytes) {
         if (pinned_size < bytes) {
             if (pinned) HIPCHK(hipHostFree(pinned));
@@ -128,6 +186,7 @@ struct Workspace {
             pinned_size = 0;
             HIPCHK(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
             pinned_size = bytes;
+            ++gen;
         }
         return pinned;
     }
@@ -139,6 +198,7 @@ struct Workspace {
             size[slot] = 0;
             HIPCHK(hipMalloc(&ptr[slot], bytes));
             size[slot] = bytes;
+            ++gen;
         }
         return ptr[slot];
     }
@@ -225,6 +285,10 @@ struct DevIn {
     const void* wgt2;
     const std::uint8_t* mask;
     int vis_kind, wgt_kind;
+    // Gridding from values held in the records' order (complex T, one per
+    // record of the sorted set): each record's data value is formed from it
+    // when the record is staged.
+    const void* stored = nullptr;
 };
 
 // One active visibility in bucket order. v2d: (re, im) its stored value
@@ -875,10 +939,19 @@ template <class T>
 __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restrict__ recs,
                                              const std::uint32_t* s_start,
                                              const std::uint32_t* s_pre, std::uint32_t j, int p0,
-                                             int p1) {
+                                             int p1, const cx<T>* __restrict__ stored) {
     int src = 0;
     while (j >= s_pre[src + 1]) ++src;
-    Rec<T> r = recs[s_start[src] + (j - s_pre[src])];
+    const std::uint32_t at = s_start[src] + (j - s_pre[src]);
+    Rec<T> r = recs[at];
+    if (stored) {
+        // the data value of a stored value v and the record's unit value p
+        // (wgridder.rs:1432-1435): v p with p's imaginary part FixW-signed
+        const T fl = (r.idx & kFlip) ? T(-1) : T(1);
+        const cx<T> d = cmul(stored[at], cx<T>{r.re, r.im * fl});
+        r.re = d.re;
+        r.im = d.im * fl;
+    }
     int du, dv, cat;
     source_of(src, du, dv, cat);
     const int ou = int(r.pos & 255u) - (du ? g.Tu : 0) + 64;
@@ -979,14 +1052,16 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
     const unsigned pstride8 = unsigned(pstride) * unsigned(sizeof(GridAcc));
     const bool stager = threadIdx.x < kGridBatch;
     Rec<T> nxt{};
-    if (stager && jbeg + threadIdx.x < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x, p0, p1);
+    const cx<T>* stored = static_cast<const cx<T>*>(in.stored);
+    if (stager && jbeg + threadIdx.x < jend)
+        nxt = grid_fetch(g, recs, s_start, s_pre, jbeg + threadIdx.x, p0, p1, stored);
     for (std::uint32_t base = jbeg; base < jend; base += kGridBatch) {
         const int nbat = int(min(jend - base, std::uint32_t(kGridBatch)));
         __syncthreads();  // the previous batch is consumed
         if (int(threadIdx.x) < nbat) s_rec[threadIdx.x] = nxt;
         __syncthreads();
         const std::uint32_t jn = base + kGridBatch + threadIdx.x;
-        if (stager && jn < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jn, p0, p1);
+        if (stager && jn < jend) nxt = grid_fetch(g, recs, s_start, s_pre, jn, p0, p1, stored);
         if constexpr (REG && RPW == 1) {
             if (g.wgrid) {
                 // One visibility per wave: its position is wave-uniform, and
@@ -1798,6 +1873,23 @@ constexpr int degrid_threads() {
     return STAGE && sizeof(T) == 4 ? 1024 : 256;
 }
 
+// Where the degridder puts each record's value c (the prediction times the
+// record's value): mode 0, vis[index] = c (dense); mode 1, stored[pos[r]] = c
+// (the values in another sorted set's order); mode 2, the residual
+// stored[pos[r]] = d - c of the data d (dense at index, or `sorted` in that
+// order), with sum w |d - c|^2 (the data's weights, dense) per workgroup into
+// part[workgroup].
+template <class T>
+struct DegridOut {
+    cx<T>* vis = nullptr;
+    int mode = 0;
+    const std::uint32_t* pos = nullptr;
+    cx<T>* stored = nullptr;
+    DevIn data{};
+    int sorted = 0;
+    double* part = nullptr;
+};
+
 template <class T, int S, bool STAGE>
 __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, const Rec<T>* __restrict__ recs,
                                                 const std::uint32_t* __restrict__ off,
@@ -1805,7 +1897,7 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
                                                 const cx<T>* __restrict__ grid, std::uint32_t rc,
                                                 const std::uint32_t* __restrict__ item_off,
                                                 const std::uint32_t* __restrict__ order,
-                                                cx<T>* __restrict__ vis) {
+                                                DegridOut<T> o) {
     constexpr int NE = ne_of<S>();
     __shared__ T coef[2 * NE * S];
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1871,6 +1963,7 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
     const std::uint32_t total = b - a;
     const std::uint32_t part = (total + blockDim.x - 1) / blockDim.x;
     const std::uint32_t j0 = a + threadIdx.x * part, j1 = min(b, j0 + part);
+    double mis = 0.0;
     for (std::uint32_t r = j0; r < j1; ++r) {
         const Rec<T> rc_ = recs[r];
         const int ou = int(rc_.pos & 255u), ov = int((rc_.pos >> 8) & 255u), mp = int(rc_.pos >> 16);
@@ -1922,7 +2015,35 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
             res = cmul(out, cx<T>{rc_.re, rc_.im});
         else
             res = {out.re * rc_.re, out.im * rc_.re};
-        vis[rc_.idx & ~kFlip] = res;
+        const std::uint32_t idx = rc_.idx & ~kFlip;
+        if (o.mode == 0) {
+            o.vis[idx] = res;
+        } else {
+            const std::uint32_t k0 = o.pos[r];
+            if (o.mode == 1) {
+                o.stored[k0] = res;
+            } else {
+                const cx<T> d = vis_get<T>(o.data, o.sorted ? (long long)k0 : (long long)idx);
+                const cx<T> q = {d.re - res.re, d.im - res.im};
+                o.stored[k0] = q;
+                T wv = T(1);
+                wgt_get<T>(o.data, idx, wv);
+                mis += double(wv) * (double(q.re) * double(q.re) + double(q.im) * double(q.im));
+            }
+        }
+    }
+    if (o.mode == 2) {
+        // the workgroup's misfit: lanes, then waves
+        for (int h = 32; h > 0; h >>= 1) mis += __shfl_down(mis, h, 64);
+        __shared__ double wsum[16];
+        const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+        if (lane == 0) wsum[wave] = mis;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int k = 0; k < int(blockDim.x) / 64; ++k) t += wsum[k];
+            o.part[blockIdx.x] = t;
+        }
     }
 }
 
@@ -1941,8 +2062,7 @@ void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo&
 #define GRID_CASE(SS)                                                                           \
     case SS:                                                                                    \
         if constexpr (SS <= max_support<T>()) {                                                 \
-            HIPCHK(hipFuncSetAttribute((const void*)k_grid<T, SS, NOUT, FORM>,                  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
+            lds_limit((const void*)k_grid<T, SS, NOUT, FORM>, lds);                             \
             k_grid<T, SS, NOUT, FORM><<<nblocks, grid_threads<T>(), lds, st>>>(g, recs, off, coeff, in, G,  \
                                                                      ngroups, TvP, rc, item_off, \
                                                                      order, out);               \
@@ -1965,13 +2085,11 @@ template <class T, bool STAGE>
 void launch_degrid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
                    const Rec<T>* recs, const std::uint32_t* off, const T* coeff,
                    const cx<T>* grid, std::uint32_t rc, const std::uint32_t* item_off,
-                   const std::uint32_t* order, cx<T>* vis) {
+                   const std::uint32_t* order, const DegridOut<T>& vis) {
 #define DEGRID_CASE(SS)                                                                        \
     case SS:                                                                                   \
         if constexpr (SS <= max_support<T>()) {                                                \
-            if (STAGE)                                                                         \
-                HIPCHK(hipFuncSetAttribute((const void*)k_degrid<T, SS, STAGE>,                \
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds))); \
+            if (STAGE) lds_limit((const void*)k_degrid<T, SS, STAGE>, lds);                   \
             k_degrid<T, SS, STAGE><<<nblocks, degrid_threads<T, STAGE>(), lds, st>>>(g, recs, off, coeff, grid, rc, \
                                                               item_off, order, vis);           \
         } else {                                                                               \
@@ -2335,8 +2453,8 @@ void sort_records(Workspace& ws, const DevIn& din, const Geo& g, long long nvis,
     std::uint32_t* off = static_cast<std::uint32_t*>(ws.get(set ? kSlotOffsets2 : kSlotOffsets, sizeof(std::uint32_t) * (nkeys + 1)));
     std::uint32_t* cursor = static_cast<std::uint32_t*>(ws.get(set ? kSlotCursor2 : kSlotCursor, sizeof(std::uint32_t) * (nkeys + 1)));
     Rec<T>* recs = static_cast<Rec<T>*>(ws.get(set ? kSlotRecs2 : kSlotRecs, sizeof(Rec<T>) * std::max<long long>(nvis, 1)));
-    HIPCHK(hipMemsetAsync(counts, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
-    HIPCHK(hipMemsetAsync(cursor, 0, sizeof(std::uint32_t) * (nkeys + 1), st));
+    fill_zero(counts, sizeof(std::uint32_t) * (nkeys + 1), st);
+    fill_zero(cursor, sizeof(std::uint32_t) * (nkeys + 1), st);
     const int eblocks = int((g.n + 255) / 256);
     k_keys<T, USE_VIS><<<eblocks, 256, 0, st>>>(din, g, keys, counts);
     HIPCHK(hipGetLastError());
@@ -2416,7 +2534,7 @@ void fft_dispatch(int need, F&& f) {
 
 template <class K>
 void set_lds(K kernel, std::size_t bytes) {
-    HIPCHK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    lds_limit((const void*)kernel, bytes);
 }
 
 // Column block of the u transforms: as many columns as fit 8192 values.
@@ -2463,7 +2581,15 @@ void make_long(Workspace& ws, const Geo& g, hipStream_t st, HostKeep& keep, Long
         add(5, roots(g.nv));
     }
     if (all.empty()) return;
-    cx<T>* d = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotLong, all, st, keep));
+    // uploaded once per geometry: a call that repeats it reads no host memory
+    const std::vector<double> key{double(sizeof(T)), double(g.nu), double(g.nv), double(g.lu1), double(g.lv1)};
+    cx<T>* d;
+    if (ws.long_key == key) {
+        d = static_cast<cx<T>*>(ws.ptr[kSlotLong]);
+    } else {
+        d = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotLong, all, st, keep));
+        ws.long_key = key;
+    }
     if (g.lu1) {
         lt.tu1 = d + ofs[0];
         lt.tu2 = d + ofs[1];
@@ -2569,6 +2695,56 @@ __global__ __launch_bounds__(256) void k_residual(DevIn in, const cx<T>* pred, c
     block_sum(q, part);
 }
 
+// map[index] = k for the sorted records k.
+template <class T>
+__global__ __launch_bounds__(256) void k_index_map(const Rec<T>* __restrict__ recs, long long n,
+                                                   std::uint32_t* __restrict__ map) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) map[recs[k].idx & ~kFlip] = std::uint32_t(k);
+}
+
+// pos[r] = map[index of record r].
+template <class T>
+__global__ __launch_bounds__(256) void k_pos_of(const Rec<T>* __restrict__ recs, long long n,
+                                                const std::uint32_t* __restrict__ map,
+                                                std::uint32_t* __restrict__ pos) {
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) pos[r] = map[recs[r].idx & ~kFlip];
+}
+
+// stored[k] = d, the data of sorted record k (dense at its index, or at k
+// when `sorted`), and per block sum w |d|^2 with the data's weights (dense).
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_resid(const Rec<T>* __restrict__ recs, long long n, DevIn data,
+                                                      int sorted, cx<T>* __restrict__ stored, double* part) {
+    double q = 0.0;
+    for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const long long idx = recs[k].idx & ~kFlip;
+        const cx<T> d = vis_get<T>(data, sorted ? k : idx);
+        stored[k] = d;
+        T w = T(1);
+        wgt_get<T>(data, idx, w);
+        q += double(w) * (double(d.re) * double(d.re) + double(d.im) * double(d.im));
+    }
+    block_sum(q, part);
+}
+
+// out[k] = dense[index of sorted record k], elements of `bytes` (4, 8 or 16).
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_sorted(const Rec<T>* __restrict__ recs, long long n,
+                                                       const unsigned char* __restrict__ dense, int bytes,
+                                                       unsigned char* __restrict__ out) {
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const long long idx = recs[k].idx & ~kFlip;
+    if (bytes == 4)
+        reinterpret_cast<std::uint32_t*>(out)[k] = reinterpret_cast<const std::uint32_t*>(dense)[idx];
+    else if (bytes == 8)
+        reinterpret_cast<uint2*>(out)[k] = reinterpret_cast<const uint2*>(dense)[idx];
+    else
+        reinterpret_cast<uint4*>(out)[k] = reinterpret_cast<const uint4*>(dense)[idx];
+}
+
 // vis[i] *= w[i].
 template <class T>
 __global__ __launch_bounds__(256) void k_weigh(DevIn in, cx<T>* vis, long long n) {
@@ -2606,6 +2782,38 @@ template <class T>
 __global__ __launch_bounds__(256) void k_cg_dir(T* p, const T* r, double beta, long long n) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
         p[i] = T(double(r[i]) + beta * double(p[i]));
+}
+
+// The same steps with the scalars on the device, s = {|r|^2, <p, A p>,
+// next |r|^2}: alpha = s0 / s1 (no step where s1 <= 0, where the solve
+// stops), beta = s2 / s0.
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_step_s(T* x, T* r, const T* p, const T* ap, const double* s, long long n,
+                                                   double* part) {
+    const double alpha = s[1] > 0.0 ? s[0] / s[1] : 0.0;
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        x[i] = T(double(x[i]) + alpha * double(p[i]));
+        const T ri = T(double(r[i]) - alpha * double(ap[i]));
+        r[i] = ri;
+        q += double(ri) * double(ri);
+    }
+    block_sum(q, part);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_dir_s(T* p, const T* r, const double* s, long long n) {
+    const double beta = s[1] > 0.0 ? s[2] / s[0] : 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        p[i] = T(double(r[i]) + beta * double(p[i]));
+}
+
+// s0 <- s2 after a step; s0 <- v at the start.
+__global__ void k_cg_roll(double* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && s[1] > 0.0) s[0] = s[2];
+}
+__global__ void k_set_scalar(double* s, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *s = v;
 }
 
 // FISTA (core/src/prox_solver.rs:283-445). Per iteration the sums
@@ -2724,6 +2932,15 @@ const char* stage_timing_name(int i) {
 }
 double stage_timing_ms(int i) { return i >= 0 && i < int(g_times.size()) ? g_times[i].second : 0.0; }
 
+// The work items of one record set and tiling, kept by a prepared plan: its
+// records never change, so the item offsets and their count are formed once
+// and later calls read no count back from the device (none of them waits on
+// the host, and a stream of them can be captured into a graph).
+struct ItemCache {
+    bool valid = false;
+    long long ub = 0;
+};
+
 // vis2dirty from sorted records: tables, gridding, the transforms and the
 // sum over planes into `dirty` and/or `psf` (nx * ny each). With `form` the
 // records hold unit values and the data values are formed from din.vis while
@@ -2731,7 +2948,7 @@ double stage_timing_ms(int i) { return i >= 0 && i < int(g_times.size()) ? g_tim
 template <class T>
 void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& din,
                    const Rec<T>* recs, const std::uint32_t* off, int nout, bool form, T* dirty,
-                   T* psf, hipStream_t st, HostKeep& keep, StageTimer& timer) {
+                   T* psf, hipStream_t st, HostKeep& keep, StageTimer& timer, ItemCache* cache = nullptr) {
     const long long npix = (long long)g.nx * g.ny;
     Tables<T> tb;
     make_tables<T>(ws, G, g, st, keep, tb, 0);
@@ -2745,22 +2962,34 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
     const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
     if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
-    const int G_planes = std::max(1, std::min(P, int(grid_lds<T>() / per_plane)));
+    // at most 32 planes per item: a staged record keeps its item plane in 5 bits
+    const int G_planes = std::max(1, std::min({P, int(grid_lds<T>() / per_plane), 32}));
     const int ngroups = (P + G_planes - 1) / G_planes;
     const int ntg = g.ntu * g.ntv * ngroups;
     const std::size_t lds = per_plane * std::min(P, G_planes);
     std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
     std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
     const std::uint32_t rc = kGridChunk;
-    k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, rc, items);
-    HIPCHK(hipGetLastError());
-    exclusive_scan(ws, items, item_off, ntg + 1, st);
+    if (!cache || !cache->valid) {
+        k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, rc, items);
+        HIPCHK(hipGetLastError());
+        exclusive_scan(ws, items, item_off, ntg + 1, st);
+    }
     if (nout == 2)
         k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
     else
         k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
     HIPCHK(hipGetLastError());
-    const long long ub = read_u32(ws, item_off + ntg, st);
+    long long ub;
+    if (cache && cache->valid) {
+        ub = cache->ub;
+    } else {
+        ub = read_u32(ws, item_off + ntg, st);
+        if (cache) {
+            cache->ub = ub;
+            cache->valid = true;
+        }
+    }
     if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
     if (ub == 0) {
         // nothing to grid: every cell was cleared above
@@ -2840,8 +3069,8 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         const std::size_t npix = o.nx * o.ny;
         std::fill(info, info + kNInfo, 0.0);
         if (nvis == 0) {
-            if (dirty) HIPCHK(hipMemsetAsync(dirty, 0, sizeof(T) * npix, st));
-            if (psf) HIPCHK(hipMemsetAsync(psf, 0, sizeof(T) * npix, st));
+            if (dirty) fill_zero(dirty, sizeof(T) * npix, st);
+            if (psf) fill_zero(psf, sizeof(T) * npix, st);
             HIPCHK(hipStreamSynchronize(st));
             return 0;
         }
@@ -2863,16 +3092,17 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         return 0;
     } catch (const DevError& e) {
         err = e.msg;
-        return 2;
+        return e.status;
     }
 }
+
 
 // dirty2vis from sorted records: the corrected image, the transforms and the
 // degridding into `cvis` (zeroed by the caller; records' values multiply).
 template <class T>
 void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T>* recs,
-                     const std::uint32_t* off, const T* dirty, cx<T>* cvis, hipStream_t st,
-                     HostKeep& keep, StageTimer& timer) {
+                     const std::uint32_t* off, const T* dirty, const DegridOut<T>& outs, hipStream_t st,
+                     HostKeep& keep, StageTimer& timer, double* total = nullptr, ItemCache* cache = nullptr) {
     Tables<T> tb;
     make_tables<T>(ws, G, g, st, keep, tb, 1);
     LongTabs<T> lt;
@@ -2922,22 +3152,42 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     }
     timer.mark("fft_v");
     const int nt = g.ntu * g.ntv;
-    std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
-    std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
+    // a prepared plan's degridding items stay in slots of their own
+    std::uint32_t* items = static_cast<std::uint32_t*>(
+        ws.get(cache ? kSlotItems2 : kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
+    std::uint32_t* item_off = static_cast<std::uint32_t*>(
+        ws.get(cache ? kSlotItemOff2 : kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
     const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
     const bool staged = dl <= degrid_lds_max<T>();
     const std::uint32_t rc = staged ? degrid_chunk_staged<T>() : kDegridChunk;
-    k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, rc, items);
-    HIPCHK(hipGetLastError());
-    exclusive_scan(ws, items, item_off, nt + 1, st);
-    const long long ub = read_u32(ws, item_off + nt, st);
+    long long ub;
+    if (cache && cache->valid) {
+        ub = cache->ub;
+    } else {
+        k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, rc, items);
+        HIPCHK(hipGetLastError());
+        exclusive_scan(ws, items, item_off, nt + 1, st);
+        ub = read_u32(ws, item_off + nt, st);
+        if (cache) {
+            cache->ub = ub;
+            cache->valid = true;
+        }
+    }
     if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
-    if (staged)
+    DegridOut<T> o = outs;
+    if (o.mode == 2) o.part = static_cast<double*>(ws.get(kSlotRed, sizeof(double) * std::max<long long>(ub, 1)));
+    if (ub == 0) {
+        // nothing to degrid
+    } else if (staged)
         launch_degrid<T, true>(g.supp, int(ub), dl, st, g, recs, off, tb.coeff, grid, rc,
-                               item_off, tb.order, cvis);
+                               item_off, tb.order, o);
     else
         launch_degrid<T, false>(g.supp, int(ub), 0, st, g, recs, off, tb.coeff, grid, kDegridChunk,
-                                item_off, tb.order, cvis);
+                                item_off, tb.order, o);
+    if (o.mode == 2 && total) {
+        k_sum_parts<<<1, 256, 0, st>>>(o.part, ub, total);
+        HIPCHK(hipGetLastError());
+    }
     timer.mark("degrid");
 }
 
@@ -2960,7 +3210,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         double wlo = 0, whi = 0;
         const unsigned long long nvis = run_scan<T, false>(ws, din, g, st, wlo, whi);
         timer.mark("scan");
-        HIPCHK(hipMemsetAsync(vis, 0, sizeof(T) * 2 * in.nrow * in.nchan, st));
+        fill_zero(vis, sizeof(T) * 2 * in.nrow * in.nchan, st);
         std::fill(info, info + kNInfo, 0.0);
         if (nvis == 0) {
             HIPCHK(hipStreamSynchronize(st));
@@ -2973,7 +3223,9 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         std::uint32_t* off;
         sort_records<T, 2, false>(ws, din, g, (long long)nvis, st, &recs, &off);
         timer.mark("sort");
-        degrid_pipeline<T>(ws, G, g, recs, off, dirty, reinterpret_cast<cx<T>*>(vis), st, keep, timer);
+        DegridOut<T> o;
+        o.vis = reinterpret_cast<cx<T>*>(vis);
+        degrid_pipeline<T>(ws, G, g, recs, off, dirty, o, st, keep, timer);
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
         std::copy(G.info, G.info + kNInfo, info);
@@ -3055,26 +3307,53 @@ struct PreparedImpl final : PreparedBase {
     Rec<T>* rd = nullptr;
     std::uint32_t* od = nullptr;
     double* foc = nullptr;
+    // for each degridding record, the position of its visibility among the
+    // gridding records: the degridder writes predictions and residuals
+    // straight into values held in the gridding order
+    std::uint32_t* pos01 = nullptr;
+    ItemCache ic_grid, ic_degrid;
+    // Graphs of the plan's repeated calls, keyed by the call and its buffers
+    // (a graph holds their addresses): the first call of a key runs eagerly
+    // and fills every cache, the second is captured on the plan's own stream,
+    // later ones replay it. The caller's stream and the plan's are ordered
+    // by events.
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::map<std::vector<std::uintptr_t>, hipGraphExec_t> graphs;
+    std::map<std::vector<std::uintptr_t>, int> seen;
+    unsigned graph_gen = 0;
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+        seen.clear();
+    }
 
     ~PreparedImpl() override {
         int cur = 0;
         if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+        drop_graphs();
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (gstream) (void)hipStreamDestroy(gstream);
         for (int k = 0; k < kSlotCount; ++k)
             if (ws.ptr[k]) (void)hipFree(ws.ptr[k]);
         if (ws.pinned) (void)hipHostFree(ws.pinned);
         if (uvw) (void)hipFree(uvw);
         if (wgt) (void)hipFree(wgt);
+        if (pos01) (void)hipFree(pos01);
         if (cur != device) (void)hipSetDevice(cur);
     }
     bool single() const override { return sizeof(T) == 4; }
     std::size_t active() const override { return std::size_t(nvis); }
     std::size_t memory() const override {
-        std::size_t b = sizeof(double) * 3 * nrow + wbytes;
+        std::size_t b = sizeof(double) * 3 * nrow + wbytes + sizeof(std::uint32_t) * std::size_t(nvis);
         for (int k = 0; k < kSlotCount; ++k) b += ws.size[k];
         return b;
     }
     void release_grids() override {
         std::lock_guard<std::mutex> lk(ws.mu);
+        drop_graphs();  // they hold the addresses of the buffers freed here
         for (int k : {int(kSlotGrid), int(kSlotTmp), int(kSlotProj), int(kSlotImg), int(kSlotVbuf), int(kSlotSolve)}) {
             if (ws.ptr[k]) (void)hipFree(ws.ptr[k]);
             ws.ptr[k] = nullptr;
@@ -3102,13 +3381,88 @@ int red_blocks(long long n) {
     return int(std::max<long long>(1, std::min<long long>(kRedBlocks, (n + 255) / 256)));
 }
 
+// Graph replays of prepared calls; RANG_GRAPHS=0 turns them off (every call
+// then runs eagerly), and stage timing does too (its events are not graph
+// work).
+bool graphs_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("RANG_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on && !g_timing;
+}
+
+// Runs body(stream) for the call `key`: eagerly on `st` the first time, then
+// captured once into a graph on the plan's stream and replayed, ordered
+// after the work already queued on `st` and before what follows on it. The
+// body must make no host round trip and allocate nothing once warm (the
+// first, eager, call sizes every buffer and fills every cache).
+template <class T, class F>
+void run_graph(PreparedImpl<T>& p, const std::vector<std::uintptr_t>& key, hipStream_t st, F&& body) {
+    if (!graphs_on()) {
+        body(st);
+        return;
+    }
+    if (p.ws.gen != p.graph_gen) {
+        // a buffer a graph uses was reallocated since it was captured
+        p.drop_graphs();
+        p.graph_gen = p.ws.gen;
+    }
+    auto it = p.graphs.find(key);
+    if (it == p.graphs.end() && p.seen[key]++ == 0) {
+        body(st);
+        return;
+    }
+    if (!p.gstream) {
+        HIPCHK(hipStreamCreateWithFlags(&p.gstream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&p.ev_in, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&p.ev_out, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(p.ev_in, st));
+    HIPCHK(hipStreamWaitEvent(p.gstream, p.ev_in, 0));
+    if (it == p.graphs.end()) {
+        HIPCHK(hipStreamBeginCapture(p.gstream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            body(p.gstream);
+        } catch (...) {
+            (void)hipStreamEndCapture(p.gstream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(p.gstream, &g));
+        if (p.ws.gen != p.graph_gen) {
+            // the capture allocated (not warm after all): run it eagerly
+            (void)hipGraphDestroy(g);
+            p.drop_graphs();
+            p.graph_gen = p.ws.gen;
+            body(p.gstream);
+            HIPCHK(hipEventRecord(p.ev_out, p.gstream));
+            HIPCHK(hipStreamWaitEvent(st, p.ev_out, 0));
+            return;
+        }
+        hipGraphExec_t e = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ie);
+        it = p.graphs.emplace(key, e).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, p.gstream));
+    HIPCHK(hipEventRecord(p.ev_out, p.gstream));
+    HIPCHK(hipStreamWaitEvent(st, p.ev_out, 0));
+}
+
+std::uintptr_t up(const void* q) { return reinterpret_cast<std::uintptr_t>(q); }
+
 // vis = A dirty (no weights) at the active visibilities, zero elsewhere.
 template <class T>
 void prepared_predict(PreparedImpl<T>& p, const T* image, cx<T>* vis, hipStream_t st, HostKeep& keep,
                       StageTimer& timer) {
-    HIPCHK(hipMemsetAsync(vis, 0, sizeof(cx<T>) * p.n, st));
+    fill_zero(vis, sizeof(cx<T>) * p.n, st);
     if (p.nvis == 0) return;
-    degrid_pipeline<T>(p.ws, p.G, p.gd, p.rd, p.od, image, vis, st, keep, timer);
+    DegridOut<T> o;
+    o.vis = vis;
+    degrid_pipeline<T>(p.ws, p.G, p.gd, p.rd, p.od, image, o, st, keep, timer, nullptr, &p.ic_degrid);
 }
 
 // image = A^H W vis over the active visibilities.
@@ -3116,11 +3470,11 @@ template <class T>
 void prepared_image(PreparedImpl<T>& p, const void* vis, int vis_kind, T* image, hipStream_t st,
                     HostKeep& keep, StageTimer& timer) {
     if (p.nvis == 0) {
-        HIPCHK(hipMemsetAsync(image, 0, sizeof(T) * p.o.nx * p.o.ny, st));
+        fill_zero(image, sizeof(T) * p.o.nx * p.o.ny, st);
         return;
     }
     grid_pipeline<T>(p.ws, p.G, p.gg, p.din(vis, vis_kind), p.rg, p.og, 1, true, image, nullptr, st,
-                     keep, timer);
+                     keep, timer, &p.ic_grid);
 }
 
 constexpr int vis_kind_of(float) { return kVisC64; }
@@ -3178,6 +3532,23 @@ PreparedBase* prepared_new(const Options& o, const DevInputs& in, const double* 
             unweighted.wgt = nullptr;
             unweighted.wgt_kind = kWgtNone;
             sort_records<T, 2, false>(p->ws, d, p->gd, p->nvis, st, &p->rd, &p->od, 1, &unweighted);
+            // position of each visibility among the gridding records, through
+            // a visibility-indexed map held only while the plan is made
+            std::uint32_t* map = static_cast<std::uint32_t*>(p->ws.get(kSlotKeys, sizeof(std::uint32_t) * p->n));
+            HIPCHK(hipMalloc(&p->pos01, sizeof(std::uint32_t) * std::size_t(p->nvis)));
+            const unsigned nb = blocks_of((p->nvis + 255) / 256, 256);
+            k_index_map<T><<<nb, 256, 0, st>>>(p->rg, p->nvis, map);
+            HIPCHK(hipGetLastError());
+            k_pos_of<T><<<nb, 256, 0, st>>>(p->rd, p->nvis, map, p->pos01);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        // the sort's per-visibility keys, counts and cursors are not needed again
+        for (int k : {int(kSlotKeys), int(kSlotKeys2), int(kSlotCounts), int(kSlotCounts2), int(kSlotCursor),
+                      int(kSlotCursor2)}) {
+            if (p->ws.ptr[k]) HIPCHK(hipFree(p->ws.ptr[k]));
+            p->ws.ptr[k] = nullptr;
+            p->ws.size[k] = 0;
         }
         HIPCHK(hipStreamSynchronize(st));
         status = 0;
@@ -3196,7 +3567,8 @@ int prepared_vis2dirty(PreparedBase* base, const void* vis, int vis_kind, T* dir
         std::lock_guard<std::mutex> lk(p.ws.mu);
         HostKeep keep;
         StageTimer timer(st);
-        prepared_image<T>(p, vis, vis_kind, dirty, st, keep, timer);
+        run_graph(p, {1, up(vis), std::uintptr_t(vis_kind), up(dirty)}, st,
+                  [&](hipStream_t s) { prepared_image<T>(p, vis, vis_kind, dirty, s, keep, timer); });
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
         std::fill(info, info + kNInfo, 0.0);
@@ -3217,36 +3589,12 @@ int prepared_dirty2vis(PreparedBase* base, const T* dirty, bool weighted, T* vis
         HostKeep keep;
         StageTimer timer(st);
         cx<T>* cv = reinterpret_cast<cx<T>*>(vis);
-        prepared_predict<T>(p, dirty, cv, st, keep, timer);
-        if (weighted && p.wkind != kWgtNone && p.n > 0)
-            k_weigh<T><<<red_blocks(p.n), 256, 0, st>>>(p.din(nullptr, kVisUnit), cv, p.n);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-        timer.finish();
-        return 0;
-    } catch (const DevError& e) {
-        err = e.msg;
-        return 2;
-    }
-}
-
-template <class T>
-int prepared_residual(PreparedBase* base, const void* vis, int vis_kind, const T* model, T* dirty,
-                      double* misfit, hipStream_t st, std::string& err) {
-    auto& p = *static_cast<PreparedImpl<T>*>(base);
-    try {
-        std::lock_guard<std::mutex> lk(p.ws.mu);
-        HostKeep keep;
-        StageTimer timer(st);
-        cx<T>* r = static_cast<cx<T>*>(p.ws.get(kSlotVbuf, sizeof(cx<T>) * std::max<long long>(p.n, 1)));
-        if (model) prepared_predict<T>(p, model, r, st, keep, timer);
-        const int nblk = red_blocks(p.n);
-        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
-        k_residual<T><<<nblk, 256, 0, st>>>(p.din(vis, vis_kind), model ? r : nullptr, r, p.n, part);
-        HIPCHK(hipGetLastError());
-        *misfit = sum_partials<T>(p.ws, part, nblk, st);
-        timer.mark("residual");
-        prepared_image<T>(p, r, vis_kind_of(T()), dirty, st, keep, timer);
+        run_graph(p, {2, up(dirty), std::uintptr_t(weighted), up(vis)}, st, [&](hipStream_t s) {
+            prepared_predict<T>(p, dirty, cv, s, keep, timer);
+            if (weighted && p.wkind != kWgtNone && p.n > 0)
+                k_weigh<T><<<red_blocks(p.n), 256, 0, s>>>(p.din(nullptr, kVisUnit), cv, p.n);
+            HIPCHK(hipGetLastError());
+        });
         HIPCHK(hipStreamSynchronize(st));
         timer.finish();
         return 0;
@@ -3258,13 +3606,136 @@ int prepared_residual(PreparedBase* base, const void* vis, int vis_kind, const T
 
 namespace {
 
-// out = A^H W A x.
+// The values one per active visibility in the gridding records' order, which
+// the fused residual and the Gram application pass from the degridder to the
+// gridder instead of a visibility-sized buffer (wgridder.rs:3300-3383).
+template <class T>
+cx<T>* stored_of(PreparedImpl<T>& p) {
+    return static_cast<cx<T>*>(p.ws.get(kSlotVbuf, sizeof(cx<T>) * std::max<long long>(p.nvis, 1)));
+}
+
+// image = A^H W v for v held in the gridding order.
+template <class T>
+void image_of_stored(PreparedImpl<T>& p, const cx<T>* stored, T* image, hipStream_t st, HostKeep& keep,
+                     StageTimer& timer) {
+    DevIn d = p.din(nullptr, kVisUnit);
+    d.stored = stored;
+    grid_pipeline<T>(p.ws, p.G, p.gg, d, p.rg, p.og, 1, false, image, nullptr, st, keep, timer, &p.ic_grid);
+}
+
+// out = A^H W A x: the degridder writes A x into the gridding order, the
+// gridder reads it from there.
+template <class T>
+void gram_body(PreparedImpl<T>& p, const T* x, T* out, hipStream_t st, HostKeep& keep, StageTimer& timer) {
+    if (p.nvis == 0) {
+        fill_zero(out, sizeof(T) * p.o.nx * p.o.ny, st);
+        return;
+    }
+    cx<T>* stored = stored_of(p);
+    DegridOut<T> o;
+    o.mode = 1;
+    o.pos = p.pos01;
+    o.stored = stored;
+    degrid_pipeline<T>(p.ws, p.G, p.gd, p.rd, p.od, x, o, st, keep, timer, nullptr, &p.ic_degrid);
+    image_of_stored<T>(p, stored, out, st, keep, timer);
+}
+
 template <class T>
 void gram_apply(PreparedImpl<T>& p, const T* x, T* out, hipStream_t st, HostKeep& keep, StageTimer& timer) {
-    cx<T>* v = static_cast<cx<T>*>(p.ws.get(kSlotVbuf, sizeof(cx<T>) * std::max<long long>(p.n, 1)));
-    prepared_predict<T>(p, x, v, st, keep, timer);
-    prepared_image<T>(p, v, vis_kind_of(T()), out, st, keep, timer);
+    run_graph(p, {3, up(x), up(out)}, st, [&](hipStream_t s) { gram_body<T>(p, x, out, s, keep, timer); });
 }
+
+// Prepared::residual / residual_sorted (wgridder.rs:3230-3255, 3300-3383):
+// the residual d - A model (d alone without a model) of the data `vis`, dense
+// or (sorted) in the gridding order, formed in the degridder's epilogue into
+// the gridding order; its misfit sum w |.|^2 with the prepared weights left
+// on the device; then its image.
+template <class T>
+int residual_impl(PreparedBase* base, const void* vis, int vis_kind, bool sorted, const T* model, T* dirty,
+                  double* misfit, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        if (p.nvis == 0) {
+            fill_zero(dirty, sizeof(T) * p.o.nx * p.o.ny, st);
+            HIPCHK(hipStreamSynchronize(st));
+            *misfit = 0.0;
+            return 0;
+        }
+        double* total = static_cast<double*>(p.ws.get(kSlotScalars, sizeof(double) * 4));
+        const std::vector<std::uintptr_t> key{4, up(vis), std::uintptr_t(vis_kind), std::uintptr_t(sorted), up(model),
+                                              up(dirty)};
+        run_graph(p, key, st, [&](hipStream_t s) {
+            cx<T>* stored = stored_of(p);
+            const DevIn data = p.din(vis, vis_kind);
+            if (model) {
+                DegridOut<T> o;
+                o.mode = 2;
+                o.pos = p.pos01;
+                o.stored = stored;
+                o.data = data;
+                o.sorted = sorted ? 1 : 0;
+                degrid_pipeline<T>(p.ws, p.G, p.gd, p.rd, p.od, model, o, s, keep, timer, total, &p.ic_degrid);
+            } else {
+                const int nblk = red_blocks(p.nvis);
+                double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+                k_gather_resid<T><<<nblk, 256, 0, s>>>(p.rg, p.nvis, data, sorted ? 1 : 0, stored, part);
+                HIPCHK(hipGetLastError());
+                k_sum_parts<<<1, 256, 0, s>>>(part, nblk, total);
+                HIPCHK(hipGetLastError());
+            }
+            timer.mark("residual");
+            image_of_stored<T>(p, stored, dirty, s, keep, timer);
+        });
+        double* h = static_cast<double*>(p.ws.host(sizeof(double)));
+        HIPCHK(hipMemcpyAsync(h, total, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        *misfit = *h;
+        timer.finish();
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+}  // namespace
+
+template <class T>
+int prepared_residual(PreparedBase* base, const void* vis, int vis_kind, const T* model, T* dirty,
+                      double* misfit, hipStream_t st, std::string& err) {
+    return residual_impl<T>(base, vis, vis_kind, false, model, dirty, misfit, st, err);
+}
+
+template <class T>
+int prepared_residual_sorted(PreparedBase* base, const void* vis, int vis_kind, const T* model, T* dirty,
+                             double* misfit, hipStream_t st, std::string& err) {
+    return residual_impl<T>(base, vis, vis_kind, true, model, dirty, misfit, st, err);
+}
+
+// Prepared::sorted (wgridder.rs:3269-3286): a dense nrow x nchan array at the
+// active visibilities, in the gridding order of the residual's values.
+template <class T>
+int prepared_sorted(PreparedBase* base, const void* dense, int bytes, void* out, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        if (p.nvis > 0) {
+            k_gather_sorted<T><<<blocks_of((p.nvis + 255) / 256, 256), 256, 0, st>>>(
+                p.rg, p.nvis, static_cast<const unsigned char*>(dense), bytes, static_cast<unsigned char*>(out));
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+namespace {
 
 template <class T>
 double dot(PreparedImpl<T>& p, const T* x, const T* y, long long n, hipStream_t st) {
@@ -3308,25 +3779,37 @@ int prepared_cg(PreparedBase* base, const T* rhs, T* x, int iterations, double t
         const long long m = (long long)p.o.nx * p.o.ny;
         T* work = static_cast<T*>(p.ws.get(kSlotSolve, sizeof(T) * 3 * std::max<long long>(m, 1)));
         T *r = work, *dir = work + m, *ap = work + 2 * m;
+        double* sc = static_cast<double*>(p.ws.get(kSlotScalars, sizeof(double) * 4));
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+        double* hb = static_cast<double*>(p.ws.host(sizeof(double) * kRedBlocks));
         const double norm_d = std::sqrt(dot<T>(p, rhs, rhs, m, st));
-        HIPCHK(hipMemsetAsync(x, 0, sizeof(T) * m, st));
+        fill_zero(x, sizeof(T) * m, st);
         HIPCHK(hipMemcpyAsync(r, rhs, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(dir, rhs, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
         double rr = norm_d * norm_d;
+        k_set_scalar<<<1, 64, 0, st>>>(sc, rr);
+        HIPCHK(hipGetLastError());
         int k = 0;
-        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
         const int nblk = red_blocks(m);
+        // one iteration: A p, <p, A p>, the step and |r|^2, the direction, on
+        // the device; the two scalars the host tests come back in one copy
+        auto iteration = [&](hipStream_t s) {
+            gram_body<T>(p, dir, ap, s, keep, timer);
+            k_dot<T><<<nblk, 256, 0, s>>>(dir, ap, m, part);
+            k_sum_parts<<<1, 256, 0, s>>>(part, nblk, sc + 1);
+            k_cg_step_s<T><<<nblk, 256, 0, s>>>(x, r, dir, ap, sc, m, part);
+            k_sum_parts<<<1, 256, 0, s>>>(part, nblk, sc + 2);
+            k_cg_dir_s<T><<<nblk, 256, 0, s>>>(dir, r, sc, m);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hb, sc + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            k_cg_roll<<<1, 64, 0, s>>>(sc);
+            HIPCHK(hipGetLastError());
+        };
         while (k < iterations && std::sqrt(rr) > tolerance * norm_d) {
-            gram_apply<T>(p, dir, ap, st, keep, timer);
-            const double pap = dot<T>(p, dir, ap, m, st);
+            run_graph(p, {5, up(x)}, st, iteration);
+            HIPCHK(hipStreamSynchronize(st));
+            const double pap = hb[0], rr_next = hb[1];
             if (!(pap > 0.0)) break;
-            const double alpha = rr / pap;
-            k_cg_step<T><<<nblk, 256, 0, st>>>(x, r, dir, ap, alpha, m, part);
-            HIPCHK(hipGetLastError());
-            const double rr_next = sum_partials<T>(p.ws, part, nblk, st);
-            const double beta = rr_next / rr;
-            k_cg_dir<T><<<nblk, 256, 0, st>>>(dir, r, beta, m);
-            HIPCHK(hipGetLastError());
             rr = rr_next;
             ++k;
         }
@@ -3340,6 +3823,67 @@ int prepared_cg(PreparedBase* base, const T* rhs, T* x, int iterations, double t
         return 2;
     }
 }
+
+// Preconditioned conjugate gradients for A^H W A x = rhs from zero, with a
+// symmetric positive definite z = M r: the CG of clean.rs:263-306 with the
+// search direction built from z (alpha = <r, z> / <p, A p>, beta = <r', z'> /
+// <r, z>) and the same stopping rule on the unpreconditioned residual,
+// |r| <= tolerance |rhs|, so the counts of Gram applications compare.
+template <class T>
+int prepared_pcg(PreparedBase* base, const std::function<void(const T*, T*, hipStream_t)>& precond,
+                 const T* rhs, T* x, int iterations, double tolerance, int* used, double* rel,
+                 hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        const long long m = (long long)p.o.nx * p.o.ny;
+        T* work = static_cast<T*>(p.ws.get(kSlotSolve, sizeof(T) * 4 * std::max<long long>(m, 1)));
+        T *r = work, *dir = work + m, *ap = work + 2 * m, *z = work + 3 * m;
+        const double norm_d = std::sqrt(dot<T>(p, rhs, rhs, m, st));
+        fill_zero(x, sizeof(T) * m, st);
+        HIPCHK(hipMemcpyAsync(r, rhs, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        precond(r, z, st);
+        HIPCHK(hipMemcpyAsync(dir, z, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        double rz = dot<T>(p, r, z, m, st);
+        double rr = norm_d * norm_d;
+        int k = 0;
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kRedBlocks));
+        const int nblk = red_blocks(m);
+        while (k < iterations && std::sqrt(rr) > tolerance * norm_d) {
+            if (!(rz > 0.0)) break;
+            gram_apply<T>(p, dir, ap, st, keep, timer);
+            const double pap = dot<T>(p, dir, ap, m, st);
+            ++k;
+            if (!(pap > 0.0)) break;
+            const double alpha = rz / pap;
+            k_cg_step<T><<<nblk, 256, 0, st>>>(x, r, dir, ap, alpha, m, part);
+            HIPCHK(hipGetLastError());
+            rr = sum_partials<T>(p.ws, part, nblk, st);
+            if (std::sqrt(rr) <= tolerance * norm_d) break;
+            precond(r, z, st);
+            const double rz_next = dot<T>(p, r, z, m, st);
+            const double beta = rz_next / rz;
+            k_cg_dir<T><<<nblk, 256, 0, st>>>(dir, z, beta, m);
+            HIPCHK(hipGetLastError());
+            rz = rz_next;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        *used = k;
+        *rel = norm_d > 0.0 ? std::sqrt(rr) / norm_d : 0.0;
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+template int prepared_pcg<float>(PreparedBase*, const std::function<void(const float*, float*, hipStream_t)>&,
+                                 const float*, float*, int, double, int*, double*, hipStream_t, std::string&);
+template int prepared_pcg<double>(PreparedBase*, const std::function<void(const double*, double*, hipStream_t)>&,
+                                  const double*, double*, int, double, int*, double*, hipStream_t, std::string&);
 
 namespace {
 
@@ -3407,9 +3951,9 @@ int prepared_fista(PreparedBase* base, const T* dirty, double q0, double lam, co
         HIPCHK(hipMemcpyAsync(x, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(x_prev, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(y, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemsetAsync(y_prev, 0, sizeof(T) * m, st));
-        HIPCHK(hipMemsetAsync(gy_prev, 0, sizeof(T) * m, st));
-        HIPCHK(hipMemsetAsync(gx_prev, 0, sizeof(T) * m, st));
+        fill_zero(y_prev, sizeof(T) * m, st);
+        fill_zero(gy_prev, sizeof(T) * m, st);
+        fill_zero(gx_prev, sizeof(T) * m, st);
         const int nblk = red_blocks(m);
         double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kFistaSlots * kRedBlocks));
         RangFistaOutcome out{};
@@ -3560,6 +4104,9 @@ int prepared_power(PreparedBase* base, std::uint64_t seed, double tolerance, std
     template int prepared_dirty2vis<T>(PreparedBase*, const T*, bool, T*, hipStream_t, std::string&); \
     template int prepared_residual<T>(PreparedBase*, const void*, int, const T*, T*, double*,       \
                                       hipStream_t, std::string&);                                   \
+    template int prepared_residual_sorted<T>(PreparedBase*, const void*, int, const T*, T*, double*, \
+                                             hipStream_t, std::string&);                            \
+    template int prepared_sorted<T>(PreparedBase*, const void*, int, void*, hipStream_t, std::string&); \
     template int prepared_gram<T>(PreparedBase*, const T*, T*, hipStream_t, std::string&);         \
     template int prepared_cg<T>(PreparedBase*, const T*, T*, int, double, int*, double*, hipStream_t, \
                                 std::string&);                                                      \

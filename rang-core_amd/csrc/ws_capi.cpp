@@ -47,8 +47,22 @@ int32_t checks(const GridInputs* i, const GridOutputs* o) {
     return 0;
 }
 
-int32_t run(const GridInputs* i, GridOutputs* o, bool single, double* timings) {
+// WSClean's padded side (grid.rs:80-89) and whether the reference's GPU path
+// transforms it (radices up to 11 there: core/src/gpu/tests.rs:274-288).
+std::size_t padded(std::size_t n, double padding) {
+    const std::size_t p = std::size_t(std::ceil(double(n) * padding));
+    return p + (4 - p % 4) % 4;
+}
+bool smooth(std::size_t n) {
+    for (std::size_t q : {2, 3, 5, 7, 11})
+        while (n % q == 0) n /= q;
+    return n == 1;
+}
+
+int32_t run(const GridInputs* i, GridOutputs* o, bool single, double* timings, bool any_length) {
     if (const int32_t c = checks(i, o)) return c;
+    if (!any_length && (!smooth(padded(i->width, i->padding)) || !smooth(padded(i->height, i->padding))))
+        return fail(1, "padded image sides must be products of 2, 3, 5, 7 and 11 for the GPU transforms");
     try {
         std::string err;
         const int st = rang::ws::invert_gpu(*i, *o, single, timings, err);
@@ -63,7 +77,7 @@ int32_t run(const GridInputs* i, GridOutputs* o, bool single, double* timings) {
 extern "C" {
 
 int32_t rang_core_grid_invert(const GridInputs* inputs, GridOutputs* outputs) {
-    return run(inputs, outputs, false, nullptr);
+    return run(inputs, outputs, false, nullptr, true);
 }
 
 int32_t rang_core_grid_invert_gpu(const GridInputs* inputs, GridOutputs* outputs, int32_t precision,
@@ -74,7 +88,7 @@ int32_t rang_core_grid_invert_gpu(const GridInputs* inputs, GridOutputs* outputs
     std::fill(timings, timings + RANG_GRID_N_TIMINGS, 0.0);
     // as core/src/gpu/ffi.rs:125-136: every other failure (arguments the
     // path rejects included) is 3
-    return run(inputs, outputs, precision == 1, timings) ? 3 : 0;
+    return run(inputs, outputs, precision == 1, timings, false) ? 3 : 0;
 }
 
 int32_t rang_core_grid_kernel(size_t kernel_size, size_t oversampling, double* kernel) {

@@ -23,6 +23,7 @@
 
 #include "wg_fft.h"
 #include "ws_device.h"
+#include "ws_fft.h"
 
 namespace rang {
 namespace ws {
@@ -429,6 +430,184 @@ struct Bufs {
     }
 };
 
+// Transforms of contiguous lines of any length N in place (ws_fft.h): kind
+// 0 one line in LDS, 1 four steps of N = M1 M2, 2 Bluestein in LDS, 3
+// Bluestein over four steps of M = M1 M2.
+template <class T>
+struct LineFft {
+    int N = 0, kind = 0, sgn = 1;
+    long long M = 0;
+    int M1 = 0, M2 = 0;
+    FftDesc d, d1, d2;
+    cx<T>*tw = nullptr, *tw1 = nullptr, *tw2 = nullptr, *chirp = nullptr, *bhat = nullptr;
+    static constexpr int kBudget = 8192;  // values per workgroup in the four-step passes
+
+    static cx<T>* upload_tw(Bufs& b, const std::vector<double>& t, hipStream_t st, std::vector<std::vector<cx<T>>>& keep) {
+        keep.emplace_back(t.size() / 2);
+        auto& h = keep.back();
+        for (std::size_t q = 0; q < h.size(); ++q) h[q] = {T(t[2 * q]), T(t[2 * q + 1])};
+        cx<T>* dptr = b.get<cx<T>>(std::max<std::size_t>(h.size(), 1));
+        if (!h.empty()) WSCHK(hipMemcpyAsync(dptr, h.data(), sizeof(cx<T>) * h.size(), hipMemcpyHostToDevice, st));
+        return dptr;
+    }
+
+    void describe(int n, FftDesc& dd, cx<T>*& t, Bufs& b, hipStream_t st, std::vector<std::vector<cx<T>>>& keep) {
+        std::vector<double> tt;
+        if (!fft_describe(n, dd, tt)) throw Fail{1, "transform length not a product of 2, 3, 5, 7 and 11"};
+        t = upload_tw(b, tt, st, keep);
+    }
+
+    void setup(int n, int sign, Bufs& b, hipStream_t st, std::vector<std::vector<cx<T>>>& keep) {
+        N = n;
+        sgn = sign;
+        const int L = lds_values<T>();
+        if (is_good(N) && N <= L) {
+            kind = 0;
+            describe(N, d, tw, b, st, keep);
+            return;
+        }
+        if (is_good(N)) {
+            kind = 1;
+            M = N;
+            if (!split_two(N, L, M1, M2)) throw Fail{1, "no two-pass split of the transform length"};
+            describe(M1, d1, tw1, b, st, keep);
+            describe(M2, d2, tw2, b, st, keep);
+            return;
+        }
+        M = good_length(2LL * N - 1);
+        // the chirp c_k = exp(sgn pi i k^2 / N) and the filter b_m = conj(c_m) at +-m
+        std::vector<cx<T>> hc(N);
+        std::vector<cx<double>> hb(std::size_t(M), cx<double>{0.0, 0.0});
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (long long k = 0; k < N; ++k) {
+            const long long r = (k * k) % (2LL * N);
+            const long double a = pi * (long double)r / (long double)N;
+            const double c = double(cosl(a)), sn = double(sinl(a)) * double(sgn);
+            hc[k] = {T(c), T(sn)};
+            hb[k] = {c, -sn};
+            if (k > 0) hb[M - k] = {c, -sn};
+        }
+        keep.emplace_back(hc);
+        chirp = b.get<cx<T>>(N);
+        WSCHK(hipMemcpyAsync(chirp, keep.back().data(), sizeof(cx<T>) * N, hipMemcpyHostToDevice, st));
+        keep.emplace_back(std::size_t(M));
+        auto& hbt = keep.back();
+        for (long long q = 0; q < M; ++q) hbt[q] = {T(hb[q].re), T(hb[q].im)};
+        bhat = b.get<cx<T>>(std::size_t(M));
+        WSCHK(hipMemcpyAsync(bhat, hbt.data(), sizeof(cx<T>) * M, hipMemcpyHostToDevice, st));
+        if (M <= L) {
+            kind = 2;
+            describe(int(M), d, tw, b, st, keep);
+            forward_lds(bhat, 1, st);
+        } else {
+            kind = 3;
+            if (!split_two(M, L, M1, M2)) throw Fail{1, "no two-pass split of the chirp transform length"};
+            describe(M1, d1, tw1, b, st, keep);
+            describe(M2, d2, tw2, b, st, keep);
+            cols(bhat, 1, -1, true, st);
+            rows(bhat, 1, -1, false, st);
+        }
+        k_scale<T><<<unsigned((M + 255) / 256), 256, 0, st>>>(bhat, M, 1.0 / double(M));
+        WSCHK(hipGetLastError());
+    }
+
+    void forward_lds(cx<T>* data, long long lines, hipStream_t st) {
+        fft_sizes<T>(d.n, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * d.n;
+            set_lds(k_line_lds<T, NT, MT>, lds);
+            for (long long l0 = 0; l0 < lines; l0 += (1LL << 30)) {
+                const long long nl = std::min(lines - l0, 1LL << 30);
+                k_line_lds<T, NT, MT><<<unsigned(nl), NT, lds, st>>>(data + l0 * d.n, d, tw, -1);
+            }
+        });
+        WSCHK(hipGetLastError());
+    }
+
+    void cols(cx<T>* buf, long long lines, int s, bool twid, hipStream_t st) {
+        const int C = std::max(1, std::min(M2, kBudget / M1));
+        const long long nb = lines * ((M2 + C - 1) / C);
+        if (nb >= (1LL << 31)) throw Fail{1, "transform batch too large"};
+        fft_sizes<T>(M1 * C, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * M1 * C;
+            set_lds(k_fs_cols<T, NT, MT>, lds);
+            k_fs_cols<T, NT, MT><<<unsigned(nb), NT, lds, st>>>(buf, M1, M2, C, d1, tw1, s, twid ? 1 : 0);
+        });
+        WSCHK(hipGetLastError());
+    }
+
+    void rows(cx<T>* buf, long long lines, int s, bool twid, hipStream_t st) {
+        const int K = std::max(1, std::min(M1, kBudget / M2));
+        const long long nb = lines * ((M1 + K - 1) / K);
+        if (nb >= (1LL << 31)) throw Fail{1, "transform batch too large"};
+        fft_sizes<T>(M2 * K, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * M2 * K;
+            set_lds(k_fs_rows<T, NT, MT>, lds);
+            k_fs_rows<T, NT, MT><<<unsigned(nb), NT, lds, st>>>(buf, M1, M2, K, d2, tw2, s, twid ? 1 : 0);
+        });
+        WSCHK(hipGetLastError());
+    }
+
+    static void transpose(const cx<T>* in, cx<T>* out, long long lines, int A, int B, hipStream_t st) {
+        for (long long l0 = 0; l0 < lines; l0 += 65535) {
+            const long long nl = std::min<long long>(65535, lines - l0);
+            k_transpose<T><<<dim3((B + 31) / 32, (A + 31) / 32, unsigned(nl)), 256, 0, st>>>(
+                in + l0 * (long long)A * B, out + l0 * (long long)A * B, A, B);
+        }
+        WSCHK(hipGetLastError());
+    }
+
+    // Lines of `scratch_values` complex values of work space at least
+    // N (kind 1) or M (kind 3) times one line.
+    void apply(cx<T>* data, long long lines, cx<T>* scratch, long long scratch_values, hipStream_t st) {
+        if (kind == 0) {
+            fft_sizes<T>(N, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds = sizeof(cx<T>) * N;
+                set_lds(k_line_lds<T, NT, MT>, lds);
+                k_line_lds<T, NT, MT><<<unsigned(lines), NT, lds, st>>>(data, d, tw, sgn);
+            });
+            WSCHK(hipGetLastError());
+            return;
+        }
+        if (kind == 2) {
+            fft_sizes<T>(int(M), [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds = sizeof(cx<T>) * M;
+                set_lds(k_blue_lds<T, NT, MT>, lds);
+                k_blue_lds<T, NT, MT><<<unsigned(lines), NT, lds, st>>>(data, N, d, tw, chirp, bhat);
+            });
+            WSCHK(hipGetLastError());
+            return;
+        }
+        const long long per = std::max<long long>(1, scratch_values / M);
+        for (long long l0 = 0; l0 < lines; l0 += per) {
+            const long long nl = std::min(per, lines - l0);
+            cx<T>* x = data + l0 * N;
+            if (kind == 1) {
+                cols(x, nl, sgn, true, st);
+                rows(x, nl, sgn, false, st);
+                transpose(x, scratch, nl, M1, M2, st);
+                WSCHK(hipMemcpyAsync(x, scratch, sizeof(cx<T>) * nl * N, hipMemcpyDeviceToDevice, st));
+            } else {
+                const unsigned gb = unsigned(std::min<long long>((nl * M + 255) / 256, 65536));
+                k_blue_in<T><<<gb, 256, 0, st>>>(x, N, M, nl, chirp, scratch);
+                cols(scratch, nl, -1, true, st);
+                rows(scratch, nl, -1, false, st);
+                k_mul_spec<T><<<gb, 256, 0, st>>>(scratch, M, nl, bhat);
+                rows(scratch, nl, +1, true, st);
+                cols(scratch, nl, +1, false, st);
+                k_blue_out<T><<<gb, 256, 0, st>>>(scratch, N, M, nl, chirp, x);
+                WSCHK(hipGetLastError());
+            }
+        }
+    }
+    // work space values apply() needs for one line
+    long long scratch_per_line() const { return kind == 1 ? N : kind == 3 ? M : 0; }
+};
+
 using Clock = std::chrono::steady_clock;
 double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
 
@@ -456,10 +635,6 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     const std::size_t nrow = i.nrow, nchan = i.nchan;
     const std::size_t n = nrow * nchan;
     const std::size_t W = pad_side(i.width, i.padding), H = pad_side(i.height, i.padding);
-    FftDesc dw, dh;
-    std::vector<double> tww, twh;
-    if (!fft_describe(int(W), dw, tww) || !fft_describe(int(H), dh, twh))
-        throw Fail{1, "padded image sides must be products of 2, 3, 5, 7 and 11 for the GPU transforms"};
     Geo g{int(W), int(H), int(i.kernel_size), int(i.oversampling), int(i.n_w_layers), i.pixel_size_x,
           i.pixel_size_y, 0.0, 0.0};
     hipStream_t st = nullptr;
@@ -565,23 +740,37 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     }
     double* d_kern = b.get<double>(klen);
     WSCHK(hipMemcpyAsync(d_kern, kern.data(), sizeof(double) * klen, hipMemcpyHostToDevice, st));
-    std::vector<cx<T>> htw(tww.size() / 2), hth(twh.size() / 2);
-    for (std::size_t q = 0; q < htw.size(); ++q) htw[q] = {T(tww[2 * q]), T(tww[2 * q + 1])};
-    for (std::size_t q = 0; q < hth.size(); ++q) hth[q] = {T(twh[2 * q]), T(twh[2 * q + 1])};
-    cx<T>* d_tww = b.get<cx<T>>(htw.size());
-    cx<T>* d_twh = b.get<cx<T>>(hth.size());
-    WSCHK(hipMemcpyAsync(d_tww, htw.data(), sizeof(cx<T>) * htw.size(), hipMemcpyHostToDevice, st));
-    WSCHK(hipMemcpyAsync(d_twh, hth.data(), sizeof(cx<T>) * hth.size(), hipMemcpyHostToDevice, st));
+    // backward transforms along x (rows of W) and y (columns of H): in LDS
+    // where the side allows, else in four steps or by Bluestein (ws_fft.h)
+    std::vector<std::vector<cx<T>>> keep;
+    LineFft<T> fx, fy;
+    fx.setup(int(W), +1, b, st, keep);
+    fy.setup(int(H), +1, b, st, keep);
+    const int Cw = std::max(1, std::min(8, 8192 / int(H)));
+    // columns in place in LDS blocks when H is a good size there; otherwise
+    // each layer is transposed so that its columns become lines
+    const bool cols_direct = fy.kind == 0 && int(H) * Cw <= lds_values<T>();
+    WSCHK(hipStreamSynchronize(st));
     // batches of the layers holding samples
     std::vector<int> layers;
     for (std::size_t l = 0; l < i.n_w_layers; ++l)
         if (cnt[l]) layers.push_back(int(l));
     const long long plane = (long long)W * H;
-    const std::size_t per_layer = 2 * sizeof(cx<T>) * std::size_t(plane);
+    // a layer's dirty and PSF grids, plus a transposed copy of one grid when
+    // the columns are not transformed in place
+    const std::size_t per_layer = (cols_direct ? 2 : 3) * sizeof(cx<T>) * std::size_t(plane);
     const std::size_t budget = std::size_t(4) << 30;
     const int batch = int(std::max<std::size_t>(1, std::min<std::size_t>(layers.size(), budget / per_layer)));
     cx<T>* d_gd = b.get<cx<T>>(std::size_t(batch) * plane);
     cx<T>* d_gp = b.get<cx<T>>(std::size_t(batch) * plane);
+    cx<T>* d_tr = cols_direct ? nullptr : b.get<cx<T>>(std::size_t(batch) * plane);
+    // work space of the four-step and chirp transforms: at most 1 GiB
+    const long long per_line = std::max(fx.scratch_per_line(), fy.scratch_per_line());
+    const long long scratch_values =
+        per_line ? std::max<long long>(per_line, std::min<long long>((1LL << 30) / sizeof(cx<T>),
+                                                                     per_line * std::max<long long>(W, H) * batch))
+                 : 0;
+    cx<T>* d_fs = per_line ? b.get<cx<T>>(std::size_t(scratch_values)) : nullptr;
     double* d_sd = b.get<double>(plane);
     double* d_sp = b.get<double>(plane);
     WSCHK(hipMemsetAsync(d_sd, 0, sizeof(double) * plane, st));
@@ -590,7 +779,6 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     double* d_wl = b.get<double>(batch);
     std::vector<int> slot(i.n_w_layers, -1);
     std::vector<double> wl(batch);
-    const int Cw = std::max(1, std::min(8, 8192 / int(H)));
     const int ncb = (int(W) + Cw - 1) / Cw;
     int nbatches = 0;
     for (std::size_t l0 = 0; l0 < layers.size(); l0 += batch) {
@@ -620,18 +808,20 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
         t_grid += since(tg);
         tg = Clock::now();
         for (cx<T>* grid : {d_gd, d_gp}) {
-            fft_sizes<T>(int(W), [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                set_lds(k_rows<T, NT, MT>, sizeof(cx<T>) * W);
-                k_rows<T, NT, MT><<<(long long)nb * H, NT, sizeof(cx<T>) * W, st>>>(grid, dw, d_tww);
-            });
-            WSCHK(hipGetLastError());
-            fft_sizes<T>(int(H) * Cw, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                set_lds(k_cols<T, NT, MT>, sizeof(cx<T>) * H * Cw);
-                k_cols<T, NT, MT><<<nb * ncb, NT, sizeof(cx<T>) * H * Cw, st>>>(grid, dh, d_twh, int(W), Cw, ncb);
-            });
-            WSCHK(hipGetLastError());
+            fx.apply(grid, (long long)nb * H, d_fs, scratch_values, st);
+            if (cols_direct) {
+                fft_sizes<T>(int(H) * Cw, [&](auto nt, auto mt) {
+                    constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                    set_lds(k_cols<T, NT, MT>, sizeof(cx<T>) * H * Cw);
+                    k_cols<T, NT, MT><<<nb * ncb, NT, sizeof(cx<T>) * H * Cw, st>>>(grid, fy.d, fy.tw, int(W), Cw,
+                                                                                   ncb);
+                });
+                WSCHK(hipGetLastError());
+            } else {
+                LineFft<T>::transpose(grid, d_tr, nb, int(H), int(W), st);
+                fy.apply(d_tr, (long long)nb * W, d_fs, scratch_values, st);
+                LineFft<T>::transpose(d_tr, grid, nb, int(W), int(H), st);
+            }
         }
         WSCHK(hipStreamSynchronize(st));
         t_fft += since(tg);

@@ -160,6 +160,10 @@ SIGNATURES = {
     "rang_amd_wgridder_prepared_dirty2vis_f32": ([_vp, _vp, _i32, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_residual_f64": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_residual_f32": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_residual_sorted_f64": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_residual_sorted_f32": ([_vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_sorted_f64": ([_vp, _vp, _sz, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_sorted_f32": ([_vp, _vp, _sz, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_gram_f64": ([_vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_gram_f32": ([_vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_cg_f64": ([_vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
@@ -208,6 +212,27 @@ SIGNATURES = {
     "rang_amd_nn_grad_images": ([_vp, _vp, _vp, _i32, _i32, _i32, ctypes.c_int64, _vp], _i32),
     "rang_amd_nn_axpby_f64": ([_vp, _vp, _dp, _dp, ctypes.c_int64, _vp], _i32),
     "rang_amd_nn_axpby_f32": ([_vp, _vp, _dp, _dp, ctypes.c_int64, _vp], _i32),
+    # the normal operator without visibilities (include/rang_core_gram.h)
+    "rang_amd_wexpansion": ([_i32, _dp, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_gram_new_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _i32, _dp, _dp, _dp, _vp], _vp),
+    "rang_amd_gram_new_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _i32, _dp, _dp, _dp, _vp], _vp),
+    "rang_amd_gram_no_w_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _dp, _vp], _vp),
+    "rang_amd_gram_no_w_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _dp, _vp], _vp),
+    "rang_amd_gram_inverse_filter": ([_vp, _dp, _vp], _vp),
+    "rang_amd_gram_apply_f64": ([_vp, _vp, _vp, _vp], _i32),
+    "rang_amd_gram_apply_f32": ([_vp, _vp, _vp, _vp], _i32),
+    "rang_amd_gram_stats": ([_vp, _vp], _i32),
+    "rang_amd_gram_side": ([_vp], _sz),
+    "rang_amd_gram_free": ([_vp], None),
+    "rang_amd_wgridder_prepared_pcg_f64": ([_vp, _vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_pcg_f32": ([_vp, _vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    # imaging weights and Stokes I (include/rang_core_weights.h)
+    "rang_core_weight_grid_shape": ([_sz, _sz, _dp, _vp], _i32),
+    "rang_core_padded_size": ([_sz, _dp], _sz),
+    "rang_core_stokes_i": ([_vp, _vp], _i32),
+    "rang_core_imaging_weights": ([_vp, _vp], _i32),
+    "rang_amd_stokes_i_dev": ([_vp, _vp, _vp], _i32),
+    "rang_amd_imaging_weights_dev": ([_vp, _vp, _vp], _i32),
     "rang_amd_stage_timing": ([_i32], None),
     "rang_amd_stage_count": ([], _i32),
     "rang_amd_stage_name": ([_i32], ctypes.c_char_p),
@@ -475,3 +500,89 @@ def grid_gpu_adapter():
     if n == 0:
         raise RuntimeError("no HIP device")
     return buf.raw[:min(int(n), 255)].decode("utf-8", "replace")
+
+
+# ---------------------------------------------------------------- weights
+# The weight functions of rang._core that rang.core_weights calls
+# (core/src/ffi_weights.rs through the PyO3 module).
+
+
+class StokesIInputs(ctypes.Structure):
+    """`StokesIInputs` (core/src/ffi_weights.rs:9-19)."""
+
+    _fields_ = [("ncell", ctypes.c_size_t), ("data", ctypes.c_void_p), ("weights", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p)]
+
+
+class StokesIOutputs(ctypes.Structure):
+    """`StokesIOutputs` (core/src/ffi_weights.rs:21-27)."""
+
+    _fields_ = [("vis", ctypes.c_void_p), ("weight", ctypes.c_void_p)]
+
+
+class ImagingWeightInputs(ctypes.Structure):
+    """`ImagingWeightInputs` (core/src/ffi_weights.rs:29-55)."""
+
+    _fields_ = [("nrow", ctypes.c_size_t), ("nchan", ctypes.c_size_t), ("uvw", ctypes.c_void_p),
+                ("wavelength", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("row_selected", ctypes.c_void_p),
+                ("class_", ctypes.c_uint32), ("robust", ctypes.c_double), ("width", ctypes.c_size_t),
+                ("height", ctypes.c_size_t), ("pixel_scale_x", ctypes.c_double), ("pixel_scale_y", ctypes.c_double),
+                ("super_weight", ctypes.c_double), ("rank_filter_level", ctypes.c_double),
+                ("rank_filter_size", ctypes.c_size_t)]
+
+
+class ImagingWeightOutputs(ctypes.Structure):
+    """`ImagingWeightOutputs` (core/src/ffi_weights.rs:57-69)."""
+
+    _fields_ = [("weights", ctypes.c_void_p), ("grid", ctypes.c_void_p), ("sums", ctypes.c_void_p)]
+
+
+def weight_grid_shape(width, height, super_weight):
+    """(status, width, height) of the weight grid after the super weight."""
+    shape = np.zeros(2, np.uintp)
+    st = int(library().rang_core_weight_grid_shape(int(width), int(height), float(super_weight), _ptr(shape)))
+    return st, int(shape[0]), int(shape[1])
+
+
+def padded_size(trimmed, padding):
+    return int(library().rang_core_padded_size(int(trimmed), float(padding)))
+
+
+def stokes_i(ncell, data, weights, flags, vis, weight):
+    """`rang_core_stokes_i` into caller-owned float32 arrays; returns its status."""
+    data = _checked(data, np.float32, "data")
+    weights = _checked(weights, np.float32, "weights")
+    flags = _checked(flags, np.uint8, "flags")
+    vis = _checked(vis, np.float32, "vis")
+    weight = _checked(weight, np.float32, "weight")
+    if (data.size != 8 * ncell or weights.size != 4 * ncell or flags.size != 4 * ncell
+            or vis.size != 2 * ncell or weight.size != ncell):
+        raise ValueError("stokes I array sizes disagree")
+    inp = StokesIInputs(ncell, data.ctypes.data, weights.ctypes.data, flags.ctypes.data)
+    out = StokesIOutputs(vis.ctypes.data, weight.ctypes.data)
+    return int(library().rang_core_stokes_i(ctypes.byref(inp), ctypes.byref(out)))
+
+
+def imaging_weights(nrow, nchan, uvw, wavelength, weights, selected, class_, robust, width, height, scale_x,
+                    scale_y, super_weight, level, size, out, grid, sums):
+    """`rang_core_imaging_weights` into caller-owned arrays; returns its status."""
+    uvw = _checked(uvw, np.float64, "uvw")
+    wavelength = _checked(wavelength, np.float64, "wavelength")
+    weights = _checked(weights, np.float32, "weights")
+    selected = _checked(selected, np.uint8, "row_selected")
+    out = _checked(out, np.float32, "out")
+    sums = _checked(sums, np.float64, "sums")
+    if (uvw.size != 3 * nrow or wavelength.size != nchan or weights.size != nrow * nchan
+            or selected.size != nrow or out.size != nrow * nchan or sums.size != 4):
+        raise ValueError("imaging weight array sizes disagree")
+    g = None
+    if grid is not None:
+        g = _checked(grid, np.float64, "grid")
+        st, gw, gh = weight_grid_shape(width, height, super_weight)
+        if st or g.size != gw * (gh // 2):
+            raise ValueError("imaging weight array sizes disagree")
+    inp = ImagingWeightInputs(nrow, nchan, uvw.ctypes.data, wavelength.ctypes.data, weights.ctypes.data,
+                              selected.ctypes.data, int(class_), float(robust), int(width), int(height),
+                              float(scale_x), float(scale_y), float(super_weight), float(level), int(size))
+    o = ImagingWeightOutputs(out.ctypes.data, None if g is None else g.ctypes.data, sums.ctypes.data)
+    return int(library().rang_core_imaging_weights(ctypes.byref(inp), ctypes.byref(o)))

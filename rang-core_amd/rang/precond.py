@@ -576,3 +576,304 @@ def preconditioned_cg(measurement, precond, rhs, iterations, tolerance, *, resta
     if stats is not None:
         stats["fallbacks"] = fallbacks
     return x, used, math.sqrt(dot(true_r, true_r)) / nb
+
+
+# --------------------------------------------------------------------------
+# A global path for the network: the circulant inverse filter of the plan's
+# PSF (rang.gram.GramOperator.inverse_filter, rang-core's GramOperator::
+# inverse_filter, core/src/gridder_gram.rs:488-509) as the preconditioner's
+# linear part, and the U-Net as a learned correction to it.
+
+
+def _rms(lib, img, acc, hw, f64):
+    dot = lib.rang_amd_nn_image_dot_f64 if f64 else lib.rang_amd_nn_image_dot_f32
+    _ok(dot(_p(img), None, _p(acc), 1, hw, _stream()))
+    return math.sqrt(float(acc[0]) / hw)
+
+
+class FilteredPreconditioner:
+    """z = M r + c (N(u) - N(-u)) / 2 with M the inverse filter (symmetric
+    positive definite, global), N the network on u = (r / rms(r), PSF,
+    M r / c) and c = rms(M r). Odd and homogeneous of degree one in r;
+    with an untrained or absent network it is M."""
+
+    def __init__(self, measurement, net, filt):
+        self.m, self.net, self.filt = measurement, net, filt
+        plan = measurement.plan
+        self.real = plan.real
+        side = measurement.side
+        if net is not None and ((net.H, net.W) != (side, side) or net.batch < 2):
+            raise ValueError("the network's buffers must hold two images of the measurement's size")
+        self.hw = side * side
+        dev = plan.device
+        self.acc = torch.empty(1, dtype=torch.float64, device=dev)
+        self.scale = torch.empty(2, dtype=torch.float64, device=dev)
+        self.pair = torch.empty((2, side, side), dtype=self.real, device=dev)
+        self.mr = torch.empty((side, side), dtype=self.real, device=dev)
+        self.applies = 0
+        if net is not None:
+            inp = net.buf[0]["in"]
+            inp.zero_()
+            psf = _psf_image(measurement).to(self.real).contiguous()
+            two = torch.stack([psf, psf]).contiguous()
+            pack = _lib().rang_amd_nn_pack_f64 if self.real == torch.float64 else _lib().rang_amd_nn_pack_f32
+            _ok(pack(_p(two), None, _p(inp), CIN, 1, 2, self.hw, _stream()))
+
+    def __call__(self, r, out=None):
+        lib, st = _lib(), _stream()
+        f64 = self.real == torch.float64
+        z = out if out is not None else torch.empty_like(r)
+        self.filt.apply(r, out=self.mr)
+        self.applies += 1
+        if self.net is None:
+            z.copy_(self.mr)
+            return z
+        nr = _rms(lib, r, self.acc, self.hw, f64)
+        c = _rms(lib, self.mr, self.acc, self.hw, f64)
+        if not (nr > 0.0 and c > 0.0):
+            z.copy_(self.mr)
+            return z
+        pack = lib.rang_amd_nn_pack_f64 if f64 else lib.rang_amd_nn_pack_f32
+        inp = self.net.buf[0]["in"]
+        self.pair[0].copy_(r)
+        self.pair[1].copy_(r)
+        self.scale[0], self.scale[1] = 1.0 / nr, -1.0 / nr
+        _ok(pack(_p(self.pair), _p(self.scale), _p(inp), CIN, 0, 2, self.hw, st))
+        self.pair[0].copy_(self.mr)
+        self.pair[1].copy_(self.mr)
+        self.scale[0], self.scale[1] = 1.0 / c, -1.0 / c
+        _ok(pack(_p(self.pair), _p(self.scale), _p(inp), CIN, 2, 2, self.hw, st))
+        y = self.net.forward(2)
+        unpack = lib.rang_amd_nn_unpack_f64 if f64 else lib.rang_amd_nn_unpack_f32
+        self.scale[0], self.scale[1] = 0.5 * c, -0.5 * c
+        _ok(unpack(_p(y), COUT, 0, _p(self.scale), _p(self.pair), 2, self.hw, st))
+        axpby = lib.rang_amd_nn_axpby_f64 if f64 else lib.rang_amd_nn_axpby_f32
+        z.copy_(self.mr)
+        _ok(axpby(_p(self.pair[0]), _p(z), 1.0, 1.0, self.hw, st))
+        _ok(axpby(_p(self.pair[1]), _p(z), 1.0, 1.0, self.hw, st))
+        return z
+
+
+class FilteredTrainer:
+    """Trains the network as the correction of FilteredPreconditioner: for
+    residual images r (the operator's images G x of random skies, and
+    residuals of the filter-preconditioned CG on random right-hand sides),
+    minimise |G (M r^ + c y) - r^|^2 / (2 |r^|^2), r^ = r / rms(r), c =
+    rms(M r^), y the network's output on (r^, PSF, M r^ / c); G is `gram`,
+    an operator with apply(x, out) (the exact normal operator of the plan, or
+    the normal operator without visibilities, rang.gram.GramOperator, which
+    costs no gridding). Counts G's applications."""
+
+    def __init__(self, measurement, net, filt, gram, *, seed=7, components=(64, 24), widths=(0.5, 24.0)):
+        self.m, self.net, self.filt, self.G = measurement, net, filt, gram
+        self.side = measurement.side
+        self.hw = self.side * self.side
+        self.seed, self.components, self.widths = seed, components, widths
+        dev = net.y.device
+        B = net.batch
+        self.B = B
+        real = measurement.plan.real
+        self.real = real
+        self.x = torch.empty((B, self.side, self.side), dtype=torch.float64, device=dev)
+        self.r = torch.empty((B, self.side, self.side), dtype=real, device=dev)
+        self.mr = torch.empty_like(self.r)
+        self.y = torch.empty_like(self.r)
+        self.u = torch.empty_like(self.r)
+        self.gu = torch.empty_like(self.r)
+        self.t = torch.empty((B, self.side, self.side), dtype=torch.float64, device=dev)
+        self.s = torch.empty(B, dtype=torch.float64, device=dev)
+        self.acc = torch.empty(B, dtype=torch.float64, device=dev)
+        psf = _psf_image(measurement).to(real).contiguous()
+        inp = net.buf[0]["in"]
+        inp.zero_()
+        allpsf = psf.expand(B, -1, -1).contiguous()
+        pack = _lib().rang_amd_nn_pack_f64 if real == torch.float64 else _lib().rang_amd_nn_pack_f32
+        _ok(pack(_p(allpsf), None, _p(inp), CIN, 1, B, self.hw, _stream()))
+        self.batches = 0
+        self.gram_applies = 0
+        self.pool = []
+
+    def _g(self, src, dst):
+        for n in range(self.B):
+            self.G.apply(src[n].contiguous(), out=dst[n])
+        self.gram_applies += self.B
+
+    def collect(self, rhs_count=8, iterations=60, keep=(0, 1, 2, 4, 8, 16, 32, 59)):
+        """Residuals of CG preconditioned by the filter alone, on right-hand
+        sides G x of random skies (G the exact normal operator here, as in the
+        solve), kept at the iterations in `keep`."""
+        plan = self.m.plan
+        lib, st = _lib(), _stream()
+        f64 = self.real == torch.float64
+        idot = lib.rang_amd_nn_image_dot_f64 if f64 else lib.rang_amd_nn_image_dot_f32
+        axpby = lib.rang_amd_nn_axpby_f64 if f64 else lib.rang_amd_nn_axpby_f32
+        acc = torch.empty(1, dtype=torch.float64, device=self.x.device)
+        hw = self.hw
+
+        def dot(a, b):
+            _ok(idot(_p(a), _p(b), _p(acc), 1, hw, st))
+            return float(acc[0])
+
+        for j in range(rhs_count):
+            self._sky(1)
+            r = plan.gram(self.x[0].to(self.real).contiguous()).clone()
+            self.gram_applies += 1
+            x = torch.zeros_like(r)
+            z = self.filt.apply(r)
+            p = z.clone()
+            rz = dot(r, z)
+            for k in range(iterations):
+                if k in keep:
+                    rr = dot(r, r)
+                    self.pool.append((r * (1.0 / math.sqrt(rr / hw))).contiguous())
+                q = plan.gram(p)
+                self.gram_applies += 1
+                pq = dot(p, q)
+                if not pq > 0:
+                    break
+                a = rz / pq
+                _ok(axpby(_p(p), _p(x), a, 1.0, hw, st))
+                _ok(axpby(_p(q), _p(r), -a, 1.0, hw, st))
+                z = self.filt.apply(r)
+                rz_n = dot(r, z)
+                _ok(axpby(_p(z), _p(p), 1.0, rz_n / rz, hw, st))
+                rz = rz_n
+        return len(self.pool)
+
+    def _sky(self, count):
+        lib, st = _lib(), _stream()
+        s0 = (self.seed * 1000003 + self.batches) & 0xFFFFFFFFFFFFFFFF
+        self.batches += 1
+        half = count // 2 if count > 1 else 0
+        if half:
+            _ok(lib.rang_amd_nn_sky_f64(_p(self.x), half, self.side, self.side, self.components[0], self.widths[0],
+                                        self.widths[1], 0, s0, st))
+        _ok(lib.rang_amd_nn_sky_f64(_p(self.x[half:]), count - half, self.side, self.side, self.components[1], 0.5,
+                                    1.5, 1, s0 ^ 0x5bd1e995, st))
+
+    def step(self, lr):
+        lib, st = _lib(), _stream()
+        B, hw = self.B, self.hw
+        f64 = self.real == torch.float64
+        idot = lib.rang_amd_nn_image_dot_f64 if f64 else lib.rang_amd_nn_image_dot_f32
+        axpby = lib.rang_amd_nn_axpby_f64 if f64 else lib.rang_amd_nn_axpby_f32
+        pack = lib.rang_amd_nn_pack_f64 if f64 else lib.rang_amd_nn_pack_f32
+        unpack = lib.rang_amd_nn_unpack_f64 if f64 else lib.rang_amd_nn_unpack_f32
+        # half the batch G x of random skies, half from the pool
+        self._sky(B)
+        self._g(self.x.to(self.real), self.r)
+        if self.pool:
+            rng = np.random.default_rng(self.seed * 7919 + self.batches)
+            for n in range(B // 2, B):
+                self.r[n].copy_(self.pool[int(rng.integers(len(self.pool)))])
+        _ok(idot(_p(self.r), None, _p(self.acc), B, hw, st))
+        nr = np.sqrt(self.acc.cpu().numpy() / hw)
+        inv_r = np.where(nr > 0, 1.0 / np.where(nr > 0, nr, 1.0), 0.0)
+        for n in range(B):
+            _ok(axpby(_p(self.r[n]), _p(self.r[n]), float(inv_r[n]) - 1.0, 1.0, hw, st))  # r^ = r / rms(r)
+            self.filt.apply(self.r[n], out=self.mr[n])
+        _ok(idot(_p(self.mr), None, _p(self.acc), B, hw, st))
+        c = np.sqrt(self.acc.cpu().numpy() / hw)
+        inv_c = np.where(c > 0, 1.0 / np.where(c > 0, c, 1.0), 0.0)
+        inp = self.net.buf[0]["in"]
+        _ok(pack(_p(self.r), None, _p(inp), CIN, 0, B, hw, st))
+        self.s.copy_(torch.from_numpy(inv_c))
+        _ok(pack(_p(self.mr), _p(self.s), _p(inp), CIN, 2, B, hw, st))
+        yout = self.net.forward(B)
+        self.s.copy_(torch.from_numpy(c))
+        _ok(unpack(_p(yout), COUT, 0, _p(self.s), _p(self.y), B, hw, st))
+        # z = M r^ + c y; u = G z - r^
+        _ok(axpby(_p(self.mr), _p(self.y), 1.0, 1.0, B * hw, st))
+        self._g(self.y, self.u)
+        _ok(axpby(_p(self.r), _p(self.u), -1.0, 1.0, B * hw, st))
+        _ok(idot(_p(self.u), None, _p(self.acc), B, hw, st))
+        loss = float(self.acc.sum().cpu()) / (2.0 * hw * B)
+        # dL/dy = c G u / (hw B)
+        self._g(self.u, self.gu)
+        self.t.copy_(self.gu)
+        self.s.copy_(torch.from_numpy(c / (hw * B)))
+        _ok(lib.rang_amd_nn_grad_images(_p(self.t), _p(self.s), _p(self.net.g_y), COUT, 0, B, hw, st))
+        self.net.zero_grad()
+        self.net.backward(B)
+        self.net.adam(lr)
+        return loss
+
+    def train(self, steps, lr=1e-3, lr_end=1e-4, log=None):
+        losses = []
+        for k in range(steps):
+            a = lr * (lr_end / lr) ** (k / max(steps - 1, 1))
+            losses.append(self.step(a))
+            if log and (k % 25 == 0 or k == steps - 1):
+                log(k, losses[-1])
+        return losses
+
+
+def safeguarded_fcg(measurement, precond, fallback, rhs, iterations, tolerance, *, window=8, factor=0.5, stats=None):
+    """Flexible CG (Notay 2000, one-term) with `precond`, judged by the exact
+    normal operator: when over `window` steps the residual falls by less
+    than `factor`, or a direction is not one of descent, the rest of the
+    solve is conjugate gradients preconditioned by `fallback` (symmetric
+    positive definite; None: plain CG) from the current iterate. Returns
+    (x, Gram applications, |rhs - A^H W A x| / |rhs| from one more exact
+    application); `stats` gets the step at which the fallback took over."""
+    plan = measurement.plan
+    real = plan.real
+    lib, st = _lib(), _stream()
+    f64 = real == torch.float64
+    axpby = lib.rang_amd_nn_axpby_f64 if f64 else lib.rang_amd_nn_axpby_f32
+    idot = lib.rang_amd_nn_image_dot_f64 if f64 else lib.rang_amd_nn_image_dot_f32
+    n = rhs.numel()
+    acc = torch.empty(1, dtype=torch.float64, device=rhs.device)
+
+    def dot(a, b):
+        _ok(idot(_p(a), _p(b), _p(acc), 1, n, st))
+        return float(acc[0])
+
+    b = rhs.to(real).contiguous()
+    x = torch.zeros_like(b)
+    r = b.clone()
+    nb = math.sqrt(dot(b, b))
+    if nb == 0.0:
+        return x, 0, 0.0
+    p = torch.zeros_like(b)
+    r_prev = torch.zeros_like(b)
+    rz_prev = 1.0
+    used, k = 0, 0
+    rr = nb * nb
+    hist = [math.sqrt(rr)]
+    switched_at = None
+    pre = precond
+    while used < iterations and math.sqrt(rr) > tolerance * nb:
+        if pre is None:
+            z = r.clone()
+        else:
+            z = pre(r)
+        rz = dot(r, z)
+        if not rz > 0.0 and switched_at is None:
+            switched_at, pre, k = used, fallback, 0
+            continue
+        if k == 0:
+            p.copy_(z)
+        else:
+            beta = (rz - dot(z, r_prev)) / rz_prev if switched_at is None else rz / rz_prev
+            _ok(axpby(_p(z), _p(p), 1.0, beta, n, st))
+        q = plan.gram(p)
+        used += 1
+        pq = dot(p, q)
+        if not pq > 0.0:
+            break
+        alpha = rz / pq
+        _ok(axpby(_p(p), _p(x), alpha, 1.0, n, st))
+        r_prev.copy_(r)
+        _ok(axpby(_p(q), _p(r), -alpha, 1.0, n, st))
+        rz_prev = rz
+        rr = dot(r, r)
+        hist.append(math.sqrt(rr))
+        k += 1
+        if switched_at is None and len(hist) > window and hist[-1] > factor * hist[-1 - window]:
+            switched_at, pre, k = used, fallback, 0
+    true_r = plan.gram(x)
+    _ok(axpby(_p(b), _p(true_r), 1.0, -1.0, n, st))
+    if stats is not None:
+        stats["switched_at"] = switched_at
+    return x, used, math.sqrt(dot(true_r, true_r)) / nb

@@ -135,3 +135,36 @@ class Prepared:
         _check(self._fn("cg")(self._h, _ptr(rhs), _ptr(x), int(iterations), float(tolerance),
                                ctypes.byref(used), ctypes.byref(rel), _stream()))
         return x, used.value, rel.value
+
+    def preconditioned_cg(self, precond, rhs, iterations, tolerance, out=None):
+        """CG on A^H W A x = rhs from zero preconditioned by a symmetric
+        positive definite `precond` (a rang.gram.GramOperator, e.g. an inverse
+        filter), stopping on the same |r| / |rhs| as conjugate_gradients:
+        (x, Gram applications used, relative residual)."""
+        x = out if out is not None else self._image()
+        used = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        _check(self._fn("pcg")(self._h, precond._h, _ptr(rhs), _ptr(x), int(iterations), float(tolerance),
+                                ctypes.byref(used), ctypes.byref(rel), _stream()))
+        return x, used.value, rel.value
+
+    def sorted(self, dense):
+        """Prepared::sorted (core/src/wgridder.rs:3269-3286): `dense` (nrow x
+        nchan, 4-, 8- or 16-byte elements) at the active visibilities, in the
+        order residual_sorted reads its data."""
+        if dense.numel() != self.nrow * self.nchan or not dense.is_contiguous():
+            raise ValueError("array lengths disagree")
+        out = torch.empty(self.active(), dtype=dense.dtype, device=dense.device)
+        _check(self._fn("sorted")(self._h, _ptr(dense), dense.element_size(), _ptr(out), _stream()))
+        return out
+
+    def residual_sorted(self, vis, model=None, out=None):
+        """Prepared::residual_sorted (:3257-3267): residual() of data held only
+        for the active visibilities, in the order of sorted()."""
+        if vis.numel() != self.active():
+            raise ValueError("array lengths disagree")
+        d = out if out is not None else self._image()
+        q = ctypes.c_double(0.0)
+        _check(self._fn("residual_sorted")(self._h, _ptr(vis), _VIS_KIND[vis.dtype], _ptr(model), _ptr(d),
+                                            ctypes.byref(q), _stream()))
+        return d, q.value

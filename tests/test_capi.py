@@ -11,7 +11,8 @@ from oracle import oracle as O
 from rang import _core, core_wgridder
 
 HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h",
-           ROOT / "include" / "rang_core_precond.h"]
+           ROOT / "include" / "rang_core_precond.h", ROOT / "include" / "rang_core_gram.h",
+           ROOT / "include" / "rang_core_weights.h"]
 
 
 def declared():
@@ -21,7 +22,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 73
+    assert len(names) == 96
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name

@@ -206,3 +206,79 @@ def test_prepared_errors():
         Prepared(opts(nx, ny, dx, 1e-5), dev(uvw), dev(freq[::-1].copy()), dev(wgt), single=True)
     with pytest.raises(ValueError, match="lengths disagree"):
         Prepared(opts(nx, ny, dx, 1e-5), dev(uvw), dev(freq), dev(wgt[:-1]), single=True)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_residual_sorted_and_sorted(single):
+    # Prepared::sorted and ::residual_sorted (core/src/wgridder.rs:3257-3286):
+    # the data held only for the active visibilities, in the plan's order,
+    # give the residual and misfit of the dense call; 9,000 visibilities on a
+    # 48 x 40 image put more than one gridding item in a tile (split tiles)
+    from rang.prepared import Prepared
+
+    uvw, freq, vis, wgt, nx, ny, dx = case(single, seed=17, nrow=3000, nchan=3)
+    eps = 1e-5 if single else 1e-9
+    rng = np.random.default_rng(19)
+    model = (rng.standard_normal((nx, ny)) * (rng.uniform(size=(nx, ny)) < 0.05)).astype(
+        np.float32 if single else np.float64)
+    p = Prepared(opts(nx, ny, dx, eps), dev(uvw), dev(freq), dev(wgt), single=single)
+    sw = host(p.sorted(dev(wgt)))
+    assert sw.size == p.active() == int((wgt != 0).sum())
+    assert np.array_equal(np.sort(sw), np.sort(wgt[wgt != 0]))
+    sv = p.sorted(dev(vis))
+    assert np.array_equal(np.sort_complex(host(sv)), np.sort_complex(vis[wgt != 0]))
+    for m in (dev(model), None):
+        img, q = p.residual(dev(vis), m)
+        img_s, q_s = p.residual_sorted(sv, m)
+        assert rel_linf(host(img_s), host(img)) < TOL[single]
+        assert abs(q_s - q) <= 1e-12 * q
+    r = vis - oracle_predict(uvw, freq, model, wgt, dx, eps).astype(vis.dtype)
+    ref = oracle_image(uvw, freq, r, wgt, nx, ny, dx, eps, single)
+    img, q = p.residual(dev(vis), dev(model))
+    assert rel_linf(host(img), ref) < TOL[single]
+    q_ref = float(np.sum(wgt.astype(np.float64) * np.abs(r.astype(np.complex128)) ** 2))
+    assert abs(q - q_ref) <= TOL_Q[single] * q_ref
+    with pytest.raises(ValueError, match="disagree"):
+        p.residual_sorted(dev(vis), None)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_graph_replays_match_eager_calls(single):
+    # A prepared call's first run is eager, its second is captured into a
+    # graph and launched, later ones replay it; new contents of the same
+    # buffers, and other eager calls in between, must show in every replay.
+    from rang.prepared import Prepared
+
+    uvw, freq, vis, wgt, nx, ny, dx = case(single, seed=23, nrow=400)
+    eps = 1e-5 if single else 1e-9
+    rdt = np.float32 if single else np.float64
+    rng = np.random.default_rng(4)
+    img1, img2 = (rng.standard_normal((nx, ny)).astype(rdt) for _ in range(2))
+    p = Prepared(opts(nx, ny, dx, eps), dev(uvw), dev(freq), dev(wgt), single=single)
+    x = dev(img1)
+    out = torch.empty((nx, ny), dtype=x.dtype, device=x.device)
+
+    def ref_gram(img):
+        pred = oracle_predict(uvw, freq, img, wgt, dx, eps)
+        return oracle_image(uvw, freq, pred.astype(vis.dtype), wgt, nx, ny, dx, eps, single)
+
+    g1, g2 = ref_gram(img1), ref_gram(img2)
+    got = []
+    for k in range(6):
+        x.copy_(dev(img1 if k % 2 == 0 else img2))
+        got.append(host(p.gram(x, out=out)).copy())
+        # other work between the replays: an eager call of another key and torch's own
+        p.vis2dirty(dev(vis))
+        torch.cuda.synchronize()
+    for k, gk in enumerate(got):
+        assert rel_linf(gk, g1 if k % 2 == 0 else g2) < 10 * TOL[single], k
+    # the image direction and the residual the same way
+    v = dev(vis)
+    d = torch.empty((nx, ny), dtype=x.dtype, device=x.device)
+    ref = oracle_image(uvw, freq, vis, wgt, nx, ny, dx, eps, single)
+    for k in range(4):
+        assert rel_linf(host(p.vis2dirty(v, out=d)), ref) < TOL[single], k
+    for k in range(4):
+        img, q = p.residual(v, x, out=d)
+        img0, q0 = p.residual(v, x)
+        assert rel_linf(host(img), host(img0)) < TOL[single] and abs(q - q0) <= 1e-12 * q0, k

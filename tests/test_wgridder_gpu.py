@@ -125,6 +125,9 @@ def test_zero_weights_and_bad_input():
     dict(nx=64, ny=48, field=0.3, wmax=2000.0, nrow=200, seed=4, nchan=5),
     dict(nx=128, ny=96, field=0.05, wmax=20.0, nrow=300, seed=5, nchan=4),
     dict(nx=16, ny=16, field=0.6, wmax=3000.0, nrow=7, seed=6, nchan=3),
+    # 12,000 visibilities on a few tiles: tiles split over several gridding
+    # items, merged with global atomics into cells cleared first
+    dict(nx=48, ny=40, field=0.15, wmax=300.0, nrow=4000, seed=21, nchan=3),
 ])
 def test_matches_oracle(case, single):
     uvw, freq, vis, wgt, nx, ny, dx = ref_case(**case)

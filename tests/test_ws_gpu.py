@@ -121,9 +121,33 @@ def test_errors():
     # even kernel size
     with pytest.raises(RuntimeError, match="code 1"):
         gpu(c, device="cpu", kernel_size=6)
-    # a padded side with a prime factor above 11 (42 -> 52 = 4 x 13)
-    with pytest.raises(RuntimeError, match="products of 2, 3, 5, 7 and 11"):
-        gpu(dict(c, width=42), device="cpu")
+    # a padded side with a prime factor above 11 (42 -> 52 = 4 x 13): the GPU
+    # entry refuses it as the reference's GPU path does (core/src/gpu/tests.rs:
+    # 274-288); the CPU entry takes any side (test_any_padded_side)
+    with pytest.raises(RuntimeError, match="code 3"):
+        gpu(dict(c, width=42))
     with pytest.raises(ValueError):
         gpu(c, precision="f16")
     assert "gfx950" in __import__("rang._core", fromlist=["x"]).grid_gpu_adapter()
+
+
+@pytest.mark.parametrize("width,height,padding,lds_max", [
+    (42, 40, 1.2, None),    # 52 = 4 x 13 by 48: Bluestein in LDS along x
+    (52, 68, 1.0, None),    # 52 x 68 (4 x 17): Bluestein along both axes
+    (60, 52, 1.0, "32"),    # 60 in four steps (6 x 10), 52 by Bluestein over four steps (M = 105)
+    (48, 48, 1.0, "16"),    # 48 in four steps both ways (transposed columns)
+])
+def test_any_padded_side(width, height, padding, lds_max, monkeypatch):
+    # The reference's CPU entry transforms any side (rustfft, core/src/grid.rs:
+    # 206-259); the oracle's FFT takes any length too. RANG_WS_FFT_LDS_MAX
+    # lowers the in-LDS limit so that small sides take the four-step and chirp
+    # paths that sides above 8192 (double) and 16384 (single) take.
+    if lds_max:
+        monkeypatch.setenv("RANG_WS_FFT_LDS_MAX", lds_max)
+    print(f"[ws] {width} x {height}, padding {padding}, LDS limit {lds_max}", flush=True)
+    c = survey_case(seed=width, width=width, height=height, w_layers=6)
+    r = gpu(c, device="cpu", padding=padding)
+    ref = oracle(c, padding=padding)
+    assert rel(r["dirty"], ref["dirty"]) < TOL["f64"]
+    assert rel(r["psf"], ref["psf"]) < TOL["f64"]
+    same_stats(r["stats"], ref["stats"])
