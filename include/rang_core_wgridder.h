@@ -146,6 +146,23 @@ int32_t rang_amd_wgridder_vis2dirty_dev_f32(const WgridderOptions *opts, size_t 
                                             const void *wgt2, int32_t wgt_kind,
                                             const uint8_t *mask, float *dirty, float *psf,
                                             double *info, void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): as
+ * rang_amd_wgridder_vis2dirty_dev_f32, but every device buffer of the call
+ * (sorted records, bucket counts, tables, w-planes, transforms) is carved
+ * from one fixed block of RANG_WGRIDDER_NOALLOC_MAX_BYTES per GPU, held from
+ * the library's load until the process ends; the call allocates no device
+ * or pinned host memory. A call that needs more returns 1 with a message.
+ * Each call rewrites the block (nothing carries over); calls on one GPU are
+ * serialised by a lock and each returns after its stream is synchronised. */
+#define RANG_WGRIDDER_NOALLOC_MAX_BYTES (256u << 20)
+int32_t rang_amd_wgridder_vis2dirty_noalloc_f32(const WgridderOptions *opts, size_t nrow,
+                                                size_t nchan, const double *uvw,
+                                                const double *freq, const void *vis,
+                                                int32_t vis_kind, const void *wgt,
+                                                const void *wgt2, int32_t wgt_kind,
+                                                const uint8_t *mask, float *dirty, float *psf,
+                                                double *info, void *stream);
 int32_t rang_amd_wgridder_dirty2vis_dev_f64(const WgridderOptions *opts, size_t nrow,
                                             size_t nchan, const double *uvw, const double *freq,
                                             const double *dirty, const void *wgt,

@@ -262,7 +262,7 @@ template <class T>
 int32_t v2d_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan,
                 const double* uvw, const double* freq, const void* vis, int32_t vis_kind,
                 const void* wgt, const void* wgt2, int32_t wgt_kind, const std::uint8_t* mask,
-                T* dirty, T* psf, double* info, void* stream) {
+                T* dirty, T* psf, double* info, void* stream, bool fixed = false) {
     try {
         if (!opts) return fail(1, "null options");
         if (!uvw || !freq || !info || (!dirty && !psf)) return fail(1, "null array");
@@ -281,8 +281,16 @@ int32_t v2d_dev(const WgridderOptions* opts, std::size_t nrow, std::size_t nchan
         in.wgt2 = wgt2;
         in.wgt_kind = wgt_kind;
         in.mask = mask;
-        const int32_t status = device_vis2dirty<T>(to_options(*opts), in, fh.data(), dirty, psf, info,
-                                                   static_cast<hipStream_t>(stream), err);
+        int32_t status;
+        if constexpr (sizeof(T) == 4) {
+            status = fixed ? device_vis2dirty_fixed<T>(to_options(*opts), in, fh.data(), dirty, psf, info,
+                                                       static_cast<hipStream_t>(stream), err)
+                           : device_vis2dirty<T>(to_options(*opts), in, fh.data(), dirty, psf, info,
+                                                 static_cast<hipStream_t>(stream), err);
+        } else {
+            status = device_vis2dirty<T>(to_options(*opts), in, fh.data(), dirty, psf, info,
+                                         static_cast<hipStream_t>(stream), err);
+        }
         return status ? fail(status, err) : 0;
     } catch (const std::exception& e) {
         return fail(2, std::string("failure in the gridder: ") + e.what());
@@ -543,6 +551,16 @@ int32_t rang_amd_wgridder_vis2dirty_dev_f32(const WgridderOptions* opts, size_t 
                                             double* info, void* stream) {
     return v2d_dev<float>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind, mask,
                           dirty, psf, info, stream);
+}
+
+int32_t rang_amd_wgridder_vis2dirty_noalloc_f32(const WgridderOptions* opts, size_t nrow,
+                                                size_t nchan, const double* uvw, const double* freq,
+                                                const void* vis, int32_t vis_kind, const void* wgt,
+                                                const void* wgt2, int32_t wgt_kind,
+                                                const uint8_t* mask, float* dirty, float* psf,
+                                                double* info, void* stream) {
+    return v2d_dev<float>(opts, nrow, nchan, uvw, freq, vis, vis_kind, wgt, wgt2, wgt_kind, mask,
+                          dirty, psf, info, stream, true);
 }
 
 int32_t rang_amd_wgridder_dirty2vis_dev_f64(const WgridderOptions* opts, size_t nrow,

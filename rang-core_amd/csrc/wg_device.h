@@ -36,6 +36,14 @@ template <class T>
 int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
                      T* psf, double* info, hipStream_t stream, std::string& err);
 
+// device_vis2dirty with every device buffer of the call carved from the
+// fixed scratch block of the current device (fixed_scratch_bytes() bytes)
+// and no allocation; a call that needs more returns 1 with a message.
+template <class T>
+int device_vis2dirty_fixed(const Options& o, const DevInputs& in, const double* freq_host,
+                           T* dirty, T* psf, double* info, hipStream_t stream, std::string& err);
+std::size_t fixed_scratch_bytes();
+
 template <class T>
 int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
                      const T* dirty, T* vis, double* info, hipStream_t stream, std::string& err);

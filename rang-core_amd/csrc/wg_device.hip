@@ -163,7 +163,10 @@ enum Slot : int {
 };
 
 // Device buffers of one GPU kept between calls and grown on demand; calls on
-// one device are serialised by its mutex.
+// one device are serialised by its mutex. In arena mode (`arena` set) every
+// slot is carved from one fixed device block instead, host staging is a
+// fixed host buffer, and nothing is allocated: a request beyond the block
+// fails with status 1.
 struct Workspace {
     void* ptr[kSlotCount] = {};
     std::size_t size[kSlotCount] = {};
@@ -171,7 +174,10 @@ struct Workspace {
     // Pinned host staging for the small transfers of a call.
     void* pinned = nullptr;
     std::size_t pinned_size = 0;
+    char* arena = nullptr;
+    std::size_t arena_cap = 0, arena_used = 0;
     unsigned gen = 0;  // counts reallocations (a graph holds the old addresses)
+    alignas(64) unsigned char arena_host[8192];
     // Signature of the plan whose tables (twiddles, coefficients, screen and
     // correction tables, tile order) the table slots hold, and their layout.
     std::vector<double> table_key;
@@ -180,6 +186,10 @@ struct Workspace {
     std::vector<int> order_key[2];
     std::vector<double> long_key;
     void* host(std::size_t bytes) {
+        if (arena) {
+            if (bytes > sizeof(arena_host)) throw DevError{"host staging beyond the fixed buffer", 1};
+            return arena_host;
+        }
         if (pinned_size < bytes) {
             if (pinned) HIPCHK(hipHostFree(pinned));
             pinned = nullptr;
@@ -192,6 +202,16 @@ struct Workspace {
     }
     void* get(int slot, std::size_t bytes) {
         bytes = std::max<std::size_t>(bytes, 256);
+        if (size[slot] < bytes && arena) {
+            bytes = (bytes + 255) & ~std::size_t(255);
+            if (bytes > arena_cap - arena_used)
+                throw DevError{"the call needs more than the " + std::to_string(arena_cap >> 20) +
+                                   " MiB of fixed scratch memory",
+                               1};
+            ptr[slot] = arena + arena_used;
+            size[slot] = bytes;
+            arena_used += bytes;
+        }
         if (size[slot] < bytes) {
             if (ptr[slot]) HIPCHK(hipFree(ptr[slot]));
             ptr[slot] = nullptr;
@@ -203,6 +223,53 @@ struct Workspace {
         return ptr[slot];
     }
 };
+
+// The fixed device block of the allocation-free entry point: every GPU the
+// library runs on holds one copy from module load until the process ends.
+constexpr std::size_t kScratchBytes = std::size_t(256) << 20;
+
+}  // namespace
+}  // namespace rang
+
+static __device__ __align__(256) unsigned char scratch[rang::kScratchBytes];
+
+namespace rang {
+namespace {
+
+// Arena mode: forget every slot and cached table (contents of the block are
+// rewritten by the call that follows).
+void arena_reset(Workspace& ws) {
+    for (int k = 0; k < kSlotCount; ++k) {
+        ws.ptr[k] = nullptr;
+        ws.size[k] = 0;
+    }
+    ws.arena_used = 0;
+    ws.table_key.clear();
+    ws.foc_key.clear();
+    ws.order_key[0].clear();
+    ws.order_key[1].clear();
+    ws.long_key.clear();
+}
+
+// The workspace over the device's copy of `scratch`. Its device address is
+// read once per device, with that device current; host code passes that
+// address, never the array's name, to kernels and copies.
+Workspace& scratch_workspace(int device) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Workspace>> all;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& w = all[device];
+    if (!w) {
+        std::unique_ptr<Workspace> n(new Workspace());
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw DevError{"no device address for the fixed scratch block"};
+        n->arena = static_cast<char*>(ptr);
+        n->arena_cap = kScratchBytes;
+        w = std::move(n);
+    }
+    return *w;
+}
 
 Workspace& workspace(int device) {
     static std::mutex mu;
@@ -3045,17 +3112,19 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     }
 }
 
+// vis2dirty on the device's workspace, or (fixed) on its fixed scratch block.
 template <class T>
-int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
-                     T* psf, double* info, hipStream_t st, std::string& err) {
+int vis2dirty_on(bool fixed, const Options& o, const DevInputs& in, const double* freq_host,
+                 T* dirty, T* psf, double* info, hipStream_t st, std::string& err) {
     try {
         if (!dirty && !psf) {
             err = "null array";
             return 1;
         }
         if (!common_checks(o, in, freq_host, err)) return 1;
-        Workspace& ws = workspace(device_of());
+        Workspace& ws = fixed ? scratch_workspace(device_of()) : workspace(device_of());
         std::lock_guard<std::mutex> lk(ws.mu);
+        if (fixed) arena_reset(ws);
         HostKeep keep;
         StageTimer timer(st);
         DevIn din{in.uvw, nullptr, in.vis, in.wgt, in.wgt2, in.mask, in.vis_kind, in.wgt_kind};
@@ -3092,10 +3161,24 @@ int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_h
         return 0;
     } catch (const DevError& e) {
         err = e.msg;
+        if (e.status != 2) (void)hipStreamSynchronize(st);
         return e.status;
     }
 }
 
+template <class T>
+int device_vis2dirty(const Options& o, const DevInputs& in, const double* freq_host, T* dirty,
+                     T* psf, double* info, hipStream_t st, std::string& err) {
+    return vis2dirty_on<T>(false, o, in, freq_host, dirty, psf, info, st, err);
+}
+
+template <class T>
+int device_vis2dirty_fixed(const Options& o, const DevInputs& in, const double* freq_host,
+                           T* dirty, T* psf, double* info, hipStream_t st, std::string& err) {
+    return vis2dirty_on<T>(true, o, in, freq_host, dirty, psf, info, st, err);
+}
+
+std::size_t fixed_scratch_bytes() { return kScratchBytes; }
 
 // dirty2vis from sorted records: the corrected image, the transforms and the
 // degridding into `cvis` (zeroed by the caller; records' values multiply).
@@ -4119,6 +4202,8 @@ RANG_PREPARED_INST(float)
 RANG_PREPARED_INST(double)
 #undef RANG_PREPARED_INST
 
+template int device_vis2dirty_fixed<float>(const Options&, const DevInputs&, const double*, float*,
+                                           float*, double*, hipStream_t, std::string&);
 template int device_vis2dirty<float>(const Options&, const DevInputs&, const double*, float*,
                                      float*, double*, hipStream_t, std::string&);
 template int device_vis2dirty<double>(const Options&, const DevInputs&, const double*, double*,

@@ -144,6 +144,8 @@ SIGNATURES = {
         [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_vis2dirty_dev_f32": (
         [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_vis2dirty_noalloc_f32": (
+        [_OPT, _sz, _sz, _vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_dirty2vis_dev_f64": (
         [_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_dirty2vis_dev_f32": (

@@ -39,9 +39,13 @@ def _check(status):
 
 
 def vis2dirty(opts, uvw, freq, vis=None, wgt=None, wgt2=None, mask=None, *, single,
-              dirty=True, psf=False, out=None):
+              dirty=True, psf=False, out=None, noalloc=False):
     """Dirty image and/or PSF (torch tensors) of device visibilities. `opts`
-    is the 18-float option list of `rang._core`; returns (dirty, psf, info)."""
+    is the 18-float option list of `rang._core`; returns (dirty, psf, info).
+    noalloc (single precision): the entry point that allocates no device
+    memory and works in the library's fixed scratch block instead."""
+    if noalloc and not single:
+        raise ValueError("the allocation-free entry point is single precision")
     o = _core.options(opts)
     nrow, nchan = uvw.shape[0], freq.shape[0]
     vis_kind = _core.VIS_UNIT if vis is None else _VIS_KIND[vis.dtype]
@@ -58,6 +62,8 @@ def vis2dirty(opts, uvw, freq, vis=None, wgt=None, wgt2=None, mask=None, *, sing
     info = np.zeros(_core.N_INFO)
     lib = _core.library()
     fn = lib.rang_amd_wgridder_vis2dirty_dev_f32 if single else lib.rang_amd_wgridder_vis2dirty_dev_f64
+    if noalloc:
+        fn = lib.rang_amd_wgridder_vis2dirty_noalloc_f32
     st = fn(ctypes.byref(o), nrow, nchan, _ptr(uvw), _ptr(freq), _ptr(vis), vis_kind, _ptr(wgt),
             _ptr(wgt2), wgt_kind, _ptr(mask), _ptr(d), _ptr(p),
             ctypes.c_void_p(info.ctypes.data), _stream())
