@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 
+#include "../../include/rang_core_batch.h"
 #include "../../include/rang_core_gram.h"
 #include "../../include/rang_core_wgridder.h"
 #include "gram.h"
@@ -865,6 +866,106 @@ void rang_amd_wgridder_prepared_release_grids(RangPrepared* h) {
 void rang_amd_wgridder_prepared_free(RangPrepared* h) { delete reinterpret_cast<PreparedBase*>(h); }
 
 void rang_amd_stage_timing(int32_t on) { set_stage_timing(on != 0); }
+}  // extern "C"
+
+namespace {
+
+template <class T>
+bool batch_image_of(const RangBatchImage& r, std::size_t k, BatchImage& b, std::string& err) {
+    const std::string at = "image " + std::to_string(k) + ": ";
+    if (!r.uvw || !r.freq || !r.vis || !r.dirty) {
+        err = at + "null array";
+        return false;
+    }
+    if (!(r.vis_kind == kVisC64 || r.vis_kind == kVisC128)) {
+        err = at + "invalid visibility array or kind";
+        return false;
+    }
+    if (!kinds_ok(r.vis, r.vis_kind, r.wgt, nullptr, r.wgt_kind, err)) {
+        err = at + err;
+        return false;
+    }
+    b.o = to_options(r.opts);
+    b.in.nrow = r.nrow;
+    b.in.nchan = r.nchan;
+    b.in.uvw = r.uvw;
+    b.in.vis = r.vis;
+    b.in.vis_kind = r.vis_kind;
+    b.in.wgt = r.wgt;
+    b.in.wgt_kind = r.wgt_kind;
+    b.in.mask = r.mask;
+    b.freq = r.freq;
+    b.dirty = r.dirty;
+    b.tile = r.tile;
+    return true;
+}
+
+template <class T>
+int32_t batch_abi(RangBatchImage* images, std::size_t n, std::size_t budget, void* stream) {
+    try {
+        if (!images && n) return fail(1, "null array");
+        if (n > (std::size_t(1) << 20)) return fail(1, "too many images in one batch");
+        std::vector<BatchImage> b(n);
+        std::string err;
+        for (std::size_t k = 0; k < n; ++k)
+            if (!batch_image_of<T>(images[k], k, b[k], err)) return fail(1, err);
+        const int status = device_batch_vis2dirty<T>(b.data(), int(n), budget, static_cast<hipStream_t>(stream), err);
+        if (status) return fail(status, err);
+        for (std::size_t k = 0; k < n; ++k) std::copy(b[k].info, b[k].info + kNInfo, images[k].info);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+template <class T>
+int32_t tune_abi(RangBatchImage* image, const char* path, int32_t kernels, RangTuneReport* report, void* stream) {
+    try {
+        if (!image) return fail(1, "null array");
+        if (kernels < 1) return fail(1, "kernels must be at least 1");
+        BatchImage b;
+        std::string err;
+        if (!batch_image_of<T>(*image, 0, b, err)) return fail(1, err);
+        double ms = 0, dms = 0;
+        int cached = 0;
+        const int status =
+            device_tune<T>(b, path, kernels, &ms, &dms, &cached, static_cast<hipStream_t>(stream), err);
+        if (status) return fail(status, err);
+        image->opts.kernel_index = b.o.kernel_index;
+        image->opts.nu = b.o.nu;
+        image->opts.nv = b.o.nv;
+        image->tile = b.tile;
+        if (report) {
+            report->ms = ms;
+            report->default_ms = cached ? 0.0 : dms;
+            report->cached = cached;
+            report->reserved = 0;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rang_amd_wgridder_batch_vis2dirty_f32(RangBatchImage* images, size_t n, size_t budget_bytes, void* stream) {
+    return batch_abi<float>(images, n, budget_bytes, stream);
+}
+int32_t rang_amd_wgridder_batch_vis2dirty_f64(RangBatchImage* images, size_t n, size_t budget_bytes, void* stream) {
+    return batch_abi<double>(images, n, budget_bytes, stream);
+}
+int32_t rang_amd_wgridder_tune_f32(RangBatchImage* image, const char* cache_path, int32_t kernels,
+                                   RangTuneReport* report, void* stream) {
+    return tune_abi<float>(image, cache_path, kernels, report, stream);
+}
+int32_t rang_amd_wgridder_tune_f64(RangBatchImage* image, const char* cache_path, int32_t kernels,
+                                   RangTuneReport* report, void* stream) {
+    return tune_abi<double>(image, cache_path, kernels, report, stream);
+}
+
 int32_t rang_amd_stage_count(void) { return stage_timing_count(); }
 const char* rang_amd_stage_name(int32_t i) { return stage_timing_name(i); }
 double rang_amd_stage_ms(int32_t i) { return stage_timing_ms(i); }

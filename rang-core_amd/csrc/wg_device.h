@@ -44,6 +44,34 @@ int device_vis2dirty_fixed(const Options& o, const DevInputs& in, const double* 
                            T* dirty, T* psf, double* info, hipStream_t stream, std::string& err);
 std::size_t fixed_scratch_bytes();
 
+// One image of a sub-band batch: options, device inputs, the frequencies
+// on the host and the device image (nx * ny of the precision); `tile` the
+// gridding tile side to aim for (0: the default); info written per image.
+struct BatchImage {
+    Options o;
+    DevInputs in;
+    const double* freq = nullptr;
+    void* dirty = nullptr;
+    int tile = 0;
+    double info[kNInfo] = {};
+};
+
+// vis2dirty of every image (weighted visibilities, one output each): the
+// scans and plans of all images, then, per group of images whose buffers
+// fit `budget` bytes (largest first), their records sorted, ONE gridding
+// launch over every image's work items, and their transforms.
+template <class T>
+int device_batch_vis2dirty(BatchImage* imgs, int n, std::size_t budget, hipStream_t stream, std::string& err);
+
+// The tuner: the fastest of the cost model's `kernels` cheapest kernel rows
+// and of the gridding tile sides for one image, timed on the device and
+// kept per signature (in `path` when not null); sets im.o.kernel_index, nu,
+// nv and im.tile. ms: the chosen time, default_ms: the cost model's own
+// choice, cached: 1 when the signature was known.
+template <class T>
+int device_tune(BatchImage& im, const char* path, int kernels, double* ms, double* default_ms, int* cached,
+                hipStream_t stream, std::string& err);
+
 template <class T>
 int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_host,
                      const T* dirty, T* vis, double* info, hipStream_t stream, std::string& err);

@@ -28,13 +28,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -1037,23 +1042,31 @@ __device__ __forceinline__ Rec<T> grid_fetch(const Geo& g, const Rec<T>* __restr
     return r;
 }
 
+// The gridding workgroup's LDS besides its accumulators: the kernel's
+// coefficients for supports up to S, the item's nine source ranges and the
+// staged records.
+template <class T, int S>
+struct GridShared {
+    T coef[2 * ne_of<S>() * S];
+    std::uint32_t start[9], pre[10];
+    Rec<T> rec[kGridBatch];
+};
+
 template <class T, int S, int NOUT, bool FORM>
-__global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
-                                                       const std::uint32_t* __restrict__ off,
-                                                       const T* __restrict__ coeff, DevIn in,
-                                                       int G, int ngroups, int TvP, std::uint32_t rc,
-                                                       const std::uint32_t* __restrict__ item_off,
-                                                       const std::uint32_t* __restrict__ order,
-                                                       cx<T>* __restrict__ grid) {
+__device__ __forceinline__ void grid_item(Geo g, const Rec<T>* __restrict__ recs,
+                                          const std::uint32_t* __restrict__ off,
+                                          const T* __restrict__ coeff, DevIn in,
+                                          int G, int ngroups, int TvP, std::uint32_t rc,
+                                          const std::uint32_t* __restrict__ item_off,
+                                          const std::uint32_t* __restrict__ order,
+                                          cx<T>* __restrict__ grid, std::uint32_t item, T* coef,
+                                          std::uint32_t* s_start, std::uint32_t* s_pre, Rec<T>* s_rec) {
     constexpr int NE = ne_of<S>();
     constexpr int SS = S * S;
     constexpr int RPW = SS <= 16 ? 4 : SS <= 32 ? 2 : 1;
     constexpr int LPG = 64 / RPW;
     constexpr int CPL = (SS + LPG - 1) / LPG;
     constexpr int NT = grid_threads<T>(), NW = NT / 64;
-    __shared__ T coef[2 * NE * S];
-    __shared__ std::uint32_t s_start[9], s_pre[10];
-    __shared__ Rec<T> s_rec[kGridBatch];
     // 16-byte aligned: the dynamic segment follows the static arrays above,
     // and LDS double atomics need naturally aligned addresses.
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1061,7 +1074,7 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
     const int ntg = g.ntu * g.ntv * ngroups;
     int tg;
     std::uint32_t chunk, nchunks;
-    if (!find_item(item_off, ntg, blockIdx.x, tg, chunk, nchunks)) return;
+    if (!find_item(item_off, ntg, item, tg, chunk, nchunks)) return;
     const int rank = tg / ngroups, grp = tg - rank * ngroups;
     const int tile = int(order[rank]);
     const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
@@ -1300,6 +1313,70 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>*
                 }
             }
         }
+}
+
+template <class T, int S, int NOUT, bool FORM>
+__global__ __launch_bounds__(grid_threads<T>()) void k_grid(Geo g, const Rec<T>* __restrict__ recs,
+                                                       const std::uint32_t* __restrict__ off,
+                                                       const T* __restrict__ coeff, DevIn in,
+                                                       int G, int ngroups, int TvP, std::uint32_t rc,
+                                                       const std::uint32_t* __restrict__ item_off,
+                                                       const std::uint32_t* __restrict__ order,
+                                                       cx<T>* __restrict__ grid) {
+    __shared__ GridShared<T, S> sh;
+    grid_item<T, S, NOUT, FORM>(g, recs, off, coeff, in, G, ngroups, TvP, rc, item_off, order, grid, blockIdx.x,
+                                sh.coef, sh.start, sh.pre, sh.rec);
+}
+
+// One image's gridding in a launch that grids several (a sub-band batch):
+// its geometry (with its support), sorted records, tables and items, and its
+// first workgroup.
+template <class T>
+struct GridJob {
+    Geo g;
+    const Rec<T>* recs;
+    const std::uint32_t* off;
+    const T* coeff;
+    DevIn in;
+    int G, ngroups, TvP;
+    std::uint32_t rc;
+    const std::uint32_t* item_off;
+    const std::uint32_t* order;
+    cx<T>* grid;
+    std::uint32_t first;
+};
+
+template <class T>
+constexpr int max_support() {
+    return sizeof(T) == 4 ? 8 : 16;
+}
+
+// Every job's items in one launch: workgroup b takes item b - first of the
+// last job whose first workgroup is at most b (jobs in order of first), and
+// runs the gridding of that job's support. The LDS besides the accumulators
+// is laid out once for the largest support.
+template <class T, int NOUT, bool FORM>
+__global__ __launch_bounds__(grid_threads<T>()) void k_grid_jobs(const GridJob<T>* __restrict__ jobs, int njobs) {
+    __shared__ GridShared<T, max_support<T>()> sh;
+    int j = 0;
+    for (int k = 1; k < njobs; ++k)
+        if (jobs[k].first <= blockIdx.x) j = k;
+    const GridJob<T>& J = jobs[j];
+    const std::uint32_t item = blockIdx.x - J.first;
+#define JOB_CASE(SS)                                                                                   \
+    case SS:                                                                                           \
+        if constexpr (SS <= max_support<T>())                                                          \
+            grid_item<T, SS, NOUT, FORM>(J.g, J.recs, J.off, J.coeff, J.in, J.G, J.ngroups, J.TvP, J.rc, \
+                                         J.item_off, J.order, J.grid, item, sh.coef, sh.start, sh.pre,  \
+                                         sh.rec);                                                      \
+        break;
+    switch (J.g.supp) {
+        JOB_CASE(4) JOB_CASE(5) JOB_CASE(6) JOB_CASE(7) JOB_CASE(8) JOB_CASE(9) JOB_CASE(10) JOB_CASE(11)
+        JOB_CASE(12) JOB_CASE(13) JOB_CASE(14) JOB_CASE(15) JOB_CASE(16)
+    default:
+        break;
+    }
+#undef JOB_CASE
 }
 
 // ------------------------------------------------------------ transforms
@@ -2116,11 +2193,6 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
 
 // ------------------------------------------------------------ dispatch
 
-template <class T>
-constexpr int max_support() {
-    return sizeof(T) == 4 ? 8 : 16;
-}
-
 template <class T, int NOUT, bool FORM>
 void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
                  const Rec<T>* recs, const std::uint32_t* off, const T* coeff, const DevIn& in,
@@ -2241,6 +2313,7 @@ struct HostKeep {
     std::vector<std::vector<double>> d;
     std::vector<std::vector<float>> f;
     std::vector<std::vector<std::uint32_t>> u;
+    std::vector<std::vector<char>> b;
 };
 
 template <class P>
@@ -2442,7 +2515,7 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
 template <class T>
-bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err) {
+bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err, int tile = 0) {
     g.nx = int(G.nx);
     g.ny = int(G.ny);
     g.nu = int(G.nu);
@@ -2479,6 +2552,8 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
         const double side = std::sqrt(double(degrid_lds_max<T>()) / (double(P) * sizeof(cx<T>))) - (g.supp - 1);
         hi = std::min(48, std::max(int(side), sizeof(T) == 4 ? 8 : 16));
         target = hi;
+    } else if (tile > 0) {
+        target = std::min(tile, hi);  // the tuner's tile side
     }
     g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64), target);
     g.Tv = choose_tile(g.nv, g.supp, std::min(hi, 64), target);
@@ -3008,74 +3083,81 @@ struct ItemCache {
     long long ub = 0;
 };
 
-// vis2dirty from sorted records: tables, gridding, the transforms and the
-// sum over planes into `dirty` and/or `psf` (nx * ny each). With `form` the
-// records hold unit values and the data values are formed from din.vis while
-// gridding (one output).
+// A gridding made ready to launch: its tables, grid, plane groups and items.
 template <class T>
-void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& din,
-                   const Rec<T>* recs, const std::uint32_t* off, int nout, bool form, T* dirty,
-                   T* psf, hipStream_t st, HostKeep& keep, StageTimer& timer, ItemCache* cache = nullptr) {
-    const long long npix = (long long)g.nx * g.ny;
+struct GridPrep {
     Tables<T> tb;
-    make_tables<T>(ws, G, g, st, keep, tb, 0);
     LongTabs<T> lt;
-    make_long<T>(ws, g, st, keep, lt);
+    int P = 0, G_planes = 0, ngroups = 0, ntg = 0, TvP = 0;
+    std::size_t lds = 0;
+    std::uint32_t* item_off = nullptr;
+    long long ub = 0;  // workgroups of the gridding launch
+    cx<T>* grid = nullptr;
+};
+
+// Tables, the grid (its cells reached by several items cleared) and the
+// work items of one vis2dirty of `nout` outputs.
+template <class T>
+void grid_prepare(Workspace& ws, const Geometry& G, const Geo& g, const std::uint32_t* off, int nout,
+                  hipStream_t st, HostKeep& keep, StageTimer& timer, ItemCache* cache, GridPrep<T>& gp) {
+    make_tables<T>(ws, G, g, st, keep, gp.tb, 0);
+    make_long<T>(ws, g, st, keep, gp.lt);
     timer.mark("tables");
     const int P = g.wgrid ? g.nplanes : 1;
+    gp.P = P;
     const long long plane = (long long)g.nu * g.nv;
-    cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
+    gp.grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
     // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
-    const int TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
-    const std::size_t per_plane = std::size_t(nout) * g.Tu * TvP * 2 * sizeof(GridAcc);
+    gp.TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
+    const std::size_t per_plane = std::size_t(nout) * g.Tu * gp.TvP * 2 * sizeof(GridAcc);
     if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
     // at most 32 planes per item: a staged record keeps its item plane in 5 bits
-    const int G_planes = std::max(1, std::min({P, int(grid_lds<T>() / per_plane), 32}));
-    const int ngroups = (P + G_planes - 1) / G_planes;
-    const int ntg = g.ntu * g.ntv * ngroups;
-    const std::size_t lds = per_plane * std::min(P, G_planes);
+    gp.G_planes = std::max(1, std::min({P, int(grid_lds<T>() / per_plane), 32}));
+    gp.ngroups = (P + gp.G_planes - 1) / gp.G_planes;
+    gp.ntg = g.ntu * g.ntv * gp.ngroups;
+    gp.lds = per_plane * std::min(P, gp.G_planes);
+    const int ntg = gp.ntg;
     std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
-    std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
-    const std::uint32_t rc = kGridChunk;
+    gp.item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
     if (!cache || !cache->valid) {
-        k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, G_planes, ngroups, g.supp, rc, items);
+        k_grid_items<<<(ntg + 1 + 255) / 256, 256, 0, st>>>(g, off, gp.tb.order, gp.G_planes, gp.ngroups, g.supp,
+                                                             kGridChunk, items);
         HIPCHK(hipGetLastError());
-        exclusive_scan(ws, items, item_off, ntg + 1, st);
+        exclusive_scan(ws, items, gp.item_off, ntg + 1, st);
     }
     if (nout == 2)
-        k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
+        k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid);
     else
-        k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, item_off, tb.order, G_planes, ngroups, grid);
+        k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid);
     HIPCHK(hipGetLastError());
-    long long ub;
     if (cache && cache->valid) {
-        ub = cache->ub;
+        gp.ub = cache->ub;
     } else {
-        ub = read_u32(ws, item_off + ntg, st);
+        gp.ub = read_u32(ws, gp.item_off + ntg, st);
         if (cache) {
-            cache->ub = ub;
+            cache->ub = gp.ub;
             cache->valid = true;
         }
     }
-    if (ub >= (1LL << 31)) throw DevError{"too many gridding items"};
-    if (ub == 0) {
-        // nothing to grid: every cell was cleared above
-    } else if (nout == 2)
-        launch_grid<T, 2, false>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
-                                 ngroups, TvP, rc, item_off, tb.order, grid);
-    else if (form)
-        launch_grid<T, 1, true>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
-                                ngroups, TvP, rc, item_off, tb.order, grid);
-    else
-        launch_grid<T, 1, false>(g.supp, int(ub), lds, st, g, recs, off, tb.coeff, din, G_planes,
-                                 ngroups, TvP, rc, item_off, tb.order, grid);
-    timer.mark("grid");
+    if (gp.ub >= (1LL << 31)) throw DevError{"too many gridding items"};
+}
+
+// The transforms of the gridded planes and their corrected sum into `dirty`
+// and/or `psf`.
+template <class T>
+void grid_finish(Workspace& ws, const Geo& g, const GridPrep<T>& gp, int nout, T* dirty, T* psf, hipStream_t st,
+                 StageTimer& timer) {
+    const long long npix = (long long)g.nx * g.ny;
+    const int P = gp.P;
+    const long long plane = (long long)g.nu * g.nv;
+    const Tables<T>& tb = gp.tb;
+    const LongTabs<T>& lt = gp.lt;
     cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
     T* proj = static_cast<T*>(ws.get(kSlotProj, sizeof(T) * (long long)P * npix));
     const int C = col_block(g.nu);
     T* outs[2] = {dirty ? dirty : psf, psf};
     for (int oi = 0; oi < nout; ++oi) {
-        cx<T>* gsrc = grid + (long long)oi * P * plane;
+        cx<T>* gsrc = gp.grid + (long long)oi * P * plane;
         if (g.lv1) {
             long_rows<T, 1>(gsrc, (long long)P * g.nu, g.lv1, g.lv2, lt.v1, lt.v2, lt.tv1, lt.tv2, lt.tvn, +1,
                             tmp, g.ny, st);
@@ -3112,10 +3194,36 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     }
 }
 
+// vis2dirty from sorted records: tables, gridding, the transforms and the
+// sum over planes into `dirty` and/or `psf` (nx * ny each). With `form` the
+// records hold unit values and the data values are formed from din.vis while
+// gridding (one output).
+template <class T>
+void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& din,
+                   const Rec<T>* recs, const std::uint32_t* off, int nout, bool form, T* dirty,
+                   T* psf, hipStream_t st, HostKeep& keep, StageTimer& timer, ItemCache* cache = nullptr) {
+    GridPrep<T> gp;
+    grid_prepare<T>(ws, G, g, off, nout, st, keep, timer, cache, gp);
+    const std::uint32_t rc = kGridChunk;
+    if (gp.ub == 0) {
+        // nothing to grid: every cell was cleared above
+    } else if (nout == 2)
+        launch_grid<T, 2, false>(g.supp, int(gp.ub), gp.lds, st, g, recs, off, gp.tb.coeff, din, gp.G_planes,
+                                 gp.ngroups, gp.TvP, rc, gp.item_off, gp.tb.order, gp.grid);
+    else if (form)
+        launch_grid<T, 1, true>(g.supp, int(gp.ub), gp.lds, st, g, recs, off, gp.tb.coeff, din, gp.G_planes,
+                                gp.ngroups, gp.TvP, rc, gp.item_off, gp.tb.order, gp.grid);
+    else
+        launch_grid<T, 1, false>(g.supp, int(gp.ub), gp.lds, st, g, recs, off, gp.tb.coeff, din, gp.G_planes,
+                                 gp.ngroups, gp.TvP, rc, gp.item_off, gp.tb.order, gp.grid);
+    timer.mark("grid");
+    grid_finish<T>(ws, g, gp, nout, dirty, psf, st, timer);
+}
+
 // vis2dirty on the device's workspace, or (fixed) on its fixed scratch block.
 template <class T>
 int vis2dirty_on(bool fixed, const Options& o, const DevInputs& in, const double* freq_host,
-                 T* dirty, T* psf, double* info, hipStream_t st, std::string& err) {
+                 T* dirty, T* psf, double* info, hipStream_t st, std::string& err, int tile = 0) {
     try {
         if (!dirty && !psf) {
             err = "null array";
@@ -3146,7 +3254,7 @@ int vis2dirty_on(bool fixed, const Options& o, const DevInputs& in, const double
         Geometry G;
         if (!make_geometry(o, sizeof(T) == 4, std::size_t(nvis), wlo, whi, G, err)) return 1;
         const int nout = (dirty ? 1 : 0) + (psf ? 1 : 0);
-        if (!fill_geo<T>(G, g, nout, false, err)) return 1;
+        if (!fill_geo<T>(G, g, nout, false, err, tile)) return 1;
         Rec<T>* recs;
         std::uint32_t* off;
         if (use_vis)
@@ -3179,6 +3287,342 @@ int device_vis2dirty_fixed(const Options& o, const DevInputs& in, const double* 
 }
 
 std::size_t fixed_scratch_bytes() { return kScratchBytes; }
+
+
+// ------------------------------------------------------------ tuner
+
+namespace {
+
+struct TuneEntry {
+    long long kidx = -1;
+    std::size_t nu = 0, nv = 0;
+    int tile = 0;
+    double ms = 0, model_ms = 0;
+};
+
+struct TuneCache {
+    std::mutex mu;
+    std::map<std::string, std::map<std::string, TuneEntry>> files;  // path -> signature -> choice
+    std::map<std::string, TuneEntry>& of(const std::string& path) {
+        auto it = files.find(path);
+        if (it != files.end()) return it->second;
+        auto& m = files[path];
+        if (!path.empty()) {
+            std::ifstream f(path);
+            std::string line;
+            while (std::getline(f, line)) {
+                std::istringstream is(line);
+                std::string key;
+                TuneEntry e;
+                if (is >> key >> e.kidx >> e.nu >> e.nv >> e.tile >> e.ms >> e.model_ms) m[key] = e;
+            }
+        }
+        return m;
+    }
+};
+
+TuneCache& tune_cache() {
+    static TuneCache c;
+    return c;
+}
+
+}  // namespace
+
+// Times vis2dirty of one image over the cost model's cheapest kernel rows
+// (`kernels` of them, one per support) at the default tile, then the best
+// row over gridding tile sides 16 to 48, and keeps the fastest choice per
+// signature (precision, image, cell, centre, epsilon, w-gridding, sigma
+// range, visibility count to half a power of two, planes of the default
+// plan) in memory and, when `path` is not empty, in that file (one line per
+// signature). Sets the image's kernel_index, nu, nv and tile. A caller's
+// explicit kernel_index is kept and only the tile tuned.
+template <class T>
+int device_tune(BatchImage& im, const char* path, int kernels, double* ms, double* default_ms, int* cached,
+                hipStream_t st, std::string& err) {
+    try {
+        if (!im.dirty) {
+            err = "null array";
+            return 1;
+        }
+        if (!common_checks(im.o, im.in, im.freq, err)) return 1;
+        if (cached) *cached = 0;
+        long long nvis;
+        double wlo = 0, whi = 0;
+        {
+            Workspace& ws = workspace(device_of());
+            std::lock_guard<std::mutex> lk(ws.mu);
+            HostKeep keep;
+            DevIn din{im.in.uvw, nullptr, im.in.vis, im.in.wgt, im.in.wgt2, im.in.mask, im.in.vis_kind,
+                      im.in.wgt_kind};
+            Geo g = base_geo(im.in.nrow, im.in.nchan, im.o);
+            din.foc = upload_foc(ws, im.freq, im.in.nchan, st, keep);
+            nvis = (long long)run_scan<T, true>(ws, din, g, st, wlo, whi);
+        }
+        if (nvis == 0) return 0;
+        const bool single = sizeof(T) == 4;
+        Geometry G0;
+        if (!make_geometry(im.o, single, std::size_t(nvis), wlo, whi, G0, err)) return 1;
+        char key[512];
+        std::snprintf(key, sizeof(key), "%s:%zux%zu:%.9gx%.9g:c%.9g,%.9g:e%.3g:w%d:s%.3g-%.3g:k%lld:n%.1f:p%zu",
+                      single ? "f32" : "f64", im.o.nx, im.o.ny, im.o.pixsize_x, im.o.pixsize_y, im.o.center_x,
+                      im.o.center_y, im.o.epsilon, im.o.do_wgridding ? 1 : 0, im.o.sigma_min, im.o.sigma_max,
+                      im.o.kernel_index, std::round(2.0 * std::log2(double(nvis))) / 2.0, G0.nplanes);
+        const std::string file = path ? path : "";
+        TuneCache& tc = tune_cache();
+        std::unique_lock<std::mutex> tl(tc.mu);
+        auto& table = tc.of(file);
+        auto hit = table.find(key);
+        TuneEntry best;
+        if (hit != table.end()) {
+            best = hit->second;
+            if (cached) *cached = 1;
+        } else {
+            tl.unlock();
+            auto time_of = [&](const Options& o2, int tile, double& out) {
+                double info[kNInfo];
+                std::string e2;
+                T* d = static_cast<T*>(im.dirty);
+                double t = 1e300;
+                for (int rep = 0; rep < 3; ++rep) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    const int rc = vis2dirty_on<T>(false, o2, im.in, im.freq, d, nullptr, info, st, e2, tile);
+                    const auto t1 = std::chrono::steady_clock::now();
+                    if (rc != 0) return false;
+                    if (rep > 0) t = std::min(t, std::chrono::duration<double, std::milli>(t1 - t0).count());
+                }
+                out = t;
+                return true;
+            };
+            std::vector<KernelChoice> choices;
+            Options base = im.o;
+            if (im.o.kernel_index >= 0) {
+                choices.push_back(KernelChoice{int(im.o.kernel_index), G0.supp, G0.nu, G0.nv, 0.0});
+            } else if (!kernel_choices(im.o, single, std::size_t(nvis), wlo, whi, choices, err)) {
+                return 1;
+            }
+            const int nk = std::max(1, std::min(kernels, int(choices.size())));
+            best.ms = 1e300;
+            for (int c = 0; c < nk; ++c) {
+                Options o2 = base;
+                o2.kernel_index = choices[c].index;
+                o2.nu = choices[c].nu;
+                o2.nv = choices[c].nv;
+                double t;
+                if (!time_of(o2, 0, t)) continue;
+                if (c == 0) best.model_ms = t;
+                if (t < best.ms) {
+                    best.ms = t;
+                    best.kidx = o2.kernel_index;
+                    best.nu = o2.nu;
+                    best.nv = o2.nv;
+                    best.tile = 0;
+                }
+            }
+            if (best.kidx < 0) {
+                err = "no kernel row could be timed";
+                return 1;
+            }
+            // tile sides for the chosen row, each distinct tiling once
+            Options ob = base;
+            ob.kernel_index = best.kidx;
+            ob.nu = best.nu;
+            ob.nv = best.nv;
+            Geometry Gb;
+            if (!make_geometry(ob, single, std::size_t(nvis), wlo, whi, Gb, err)) return 1;
+            std::vector<std::pair<int, int>> seen;
+            {
+                Geo gd{};
+                if (fill_geo<T>(Gb, gd, 1, false, err, 0)) seen.emplace_back(gd.Tu, gd.Tv);
+            }
+            for (int tile : {16, 24, 40, 48}) {
+                Geo gt{};
+                std::string e3;
+                if (!fill_geo<T>(Gb, gt, 1, false, e3, tile)) continue;
+                if (std::find(seen.begin(), seen.end(), std::make_pair(gt.Tu, gt.Tv)) != seen.end()) continue;
+                seen.emplace_back(gt.Tu, gt.Tv);
+                double t;
+                if (time_of(ob, tile, t) && t < best.ms) {
+                    best.ms = t;
+                    best.tile = tile;
+                }
+            }
+            tl.lock();
+            table[key] = best;
+            if (!file.empty()) {
+                std::ofstream f(file, std::ios::app);
+                char line[256];
+                std::snprintf(line, sizeof(line), " %lld %zu %zu %d %.6g %.6g\n", best.kidx, best.nu, best.nv,
+                              best.tile, best.ms, best.model_ms);
+                f << key << line;
+            }
+        }
+        im.o.kernel_index = best.kidx;
+        im.o.nu = best.nu;
+        im.o.nv = best.nv;
+        im.tile = best.tile;
+        if (ms) *ms = best.ms;
+        if (default_ms) *default_ms = best.model_ms;
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        if (e.status != 2) (void)hipStreamSynchronize(st);
+        return e.status;
+    }
+}
+
+// ------------------------------------------------------------ sub-band batches
+
+namespace {
+
+// Workspaces of a batch: one per image of a launch group, and one for the
+// scans and the job table; kept per device between calls.
+struct BatchPool {
+    std::mutex mu;
+    Workspace control;
+    std::vector<std::unique_ptr<Workspace>> images;
+    Workspace& image(std::size_t k) {
+        while (images.size() <= k) images.emplace_back(new Workspace());
+        return *images[k];
+    }
+};
+
+BatchPool& batch_pool(int device) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<BatchPool>> all;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& p = all[device];
+    if (!p) p.reset(new BatchPool());
+    return *p;
+}
+
+// Device bytes one image's vis2dirty holds: records and their bucket
+// tables, and the grid, the half-transformed planes and the projections.
+template <class T>
+std::size_t image_bytes(const Geo& g, long long nvis) {
+    const long long P = g.wgrid ? g.nplanes : 1;
+    const long long nkeys = (long long)g.ntu * g.ntv * 4 * g.nb;
+    return std::size_t(nvis) * (sizeof(Rec<T>) + 8) + std::size_t(g.n) * 4 + std::size_t(nkeys) * 16 +
+           std::size_t(P) * ((std::size_t)g.nu * g.nv * sizeof(cx<T>) + (std::size_t)g.nu * g.ny * sizeof(cx<T>) +
+                             (std::size_t)g.nx * g.ny * sizeof(T));
+}
+
+}  // namespace
+
+template <class T>
+int device_batch_vis2dirty(BatchImage* imgs, int n, std::size_t budget, hipStream_t st, std::string& err) {
+    try {
+        if (n < 0 || (n > 0 && !imgs)) {
+            err = "null array";
+            return 1;
+        }
+        BatchPool& pool = batch_pool(device_of());
+        std::lock_guard<std::mutex> lk(pool.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        struct Plan {
+            Geometry G;
+            Geo g;
+            DevIn din;
+            long long nvis = 0;
+            std::size_t bytes = 0;
+        };
+        std::vector<Plan> plans(n);
+        // scans and plans of every image
+        for (int i = 0; i < n; ++i) {
+            BatchImage& im = imgs[i];
+            Plan& pl = plans[i];
+            if (!im.dirty) {
+                err = "image " + std::to_string(i) + ": null array";
+                return 1;
+            }
+            if (!common_checks(im.o, im.in, im.freq, err)) {
+                err = "image " + std::to_string(i) + ": " + err;
+                return 1;
+            }
+            std::fill(im.info, im.info + kNInfo, 0.0);
+            pl.din = DevIn{im.in.uvw, nullptr, im.in.vis, im.in.wgt, im.in.wgt2, im.in.mask, im.in.vis_kind,
+                           im.in.wgt_kind};
+            pl.g = base_geo(im.in.nrow, im.in.nchan, im.o);
+            pl.din.foc = upload_foc(pool.control, im.freq, im.in.nchan, st, keep);
+            double wlo = 0, whi = 0;
+            pl.nvis = (long long)run_scan<T, true>(pool.control, pl.din, pl.g, st, wlo, whi);
+            if (pl.nvis == 0) continue;
+            if (!make_geometry(im.o, sizeof(T) == 4, std::size_t(pl.nvis), wlo, whi, pl.G, err) ||
+                !fill_geo<T>(pl.G, pl.g, 1, false, err, im.tile)) {
+                err = "image " + std::to_string(i) + ": " + err;
+                return 1;
+            }
+            std::copy(pl.G.info, pl.G.info + kNInfo, im.info);
+            pl.bytes = image_bytes<T>(pl.g, pl.nvis);
+        }
+        timer.mark("scan");
+        // images without visibilities: zero images
+        std::vector<int> order;
+        for (int i = 0; i < n; ++i) {
+            if (plans[i].nvis == 0)
+                fill_zero(imgs[i].dirty, sizeof(T) * imgs[i].o.nx * imgs[i].o.ny, st);
+            else
+                order.push_back(i);
+        }
+        // largest first, so that a later group's image never needs more of a
+        // pooled workspace than the first group's image in its place
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plans[a].bytes > plans[b].bytes; });
+        std::size_t at = 0;
+        while (at < order.size()) {
+            std::size_t end = at, used = 0;
+            while (end < order.size() && (end == at || used + plans[order[end]].bytes <= budget))
+                used += plans[order[end++]].bytes;
+            std::vector<GridPrep<T>> preps(end - at);
+            std::vector<GridJob<T>> jobs;
+            std::size_t lds = 0;
+            long long total = 0;
+            for (std::size_t k = at; k < end; ++k) {
+                const int i = order[k];
+                Plan& pl = plans[i];
+                Workspace& ws = pool.image(k - at);
+                pl.din.foc = upload_foc(ws, imgs[i].freq, imgs[i].in.nchan, st, keep);
+                Rec<T>* recs;
+                std::uint32_t* off;
+                sort_records<T, 0, true>(ws, pl.din, pl.g, pl.nvis, st, &recs, &off);
+                GridPrep<T>& gp = preps[k - at];
+                grid_prepare<T>(ws, pl.G, pl.g, off, 1, st, keep, timer, nullptr, gp);
+                if (gp.ub == 0) continue;
+                if (total + gp.ub >= (1LL << 31)) throw DevError{"too many gridding items in one batch"};
+                jobs.push_back(GridJob<T>{pl.g, recs, off, gp.tb.coeff, pl.din, gp.G_planes, gp.ngroups, gp.TvP,
+                                          kGridChunk, gp.item_off, gp.tb.order, gp.grid, std::uint32_t(total)});
+                total += gp.ub;
+                lds = std::max(lds, gp.lds);
+            }
+            timer.mark("sort");
+            if (!jobs.empty()) {
+                keep.b.emplace_back(reinterpret_cast<const char*>(jobs.data()),
+                                    reinterpret_cast<const char*>(jobs.data() + jobs.size()));
+                const std::size_t jb = sizeof(GridJob<T>) * jobs.size();
+                void* dj = pool.control.get(kSlotItems, jb);
+                HIPCHK(hipMemcpyAsync(dj, keep.b.back().data(), jb, hipMemcpyHostToDevice, st));
+                lds_limit((const void*)k_grid_jobs<T, 1, false>, lds);
+                k_grid_jobs<T, 1, false><<<int(total), grid_threads<T>(), lds, st>>>(
+                    static_cast<const GridJob<T>*>(dj), int(jobs.size()));
+                HIPCHK(hipGetLastError());
+            }
+            timer.mark("grid");
+            for (std::size_t k = at; k < end; ++k) {
+                const int i = order[k];
+                grid_finish<T>(pool.image(k - at), plans[i].g, preps[k - at], 1, static_cast<T*>(imgs[i].dirty),
+                               nullptr, st, timer);
+            }
+            HIPCHK(hipStreamSynchronize(st));
+            at = end;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        timer.finish();
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        if (e.status != 2) (void)hipStreamSynchronize(st);
+        return e.status;
+    }
+}
 
 // dirty2vis from sorted records: the corrected image, the transforms and the
 // degridding into `cvis` (zeroed by the caller; records' values multiply).
@@ -4202,6 +4646,10 @@ RANG_PREPARED_INST(float)
 RANG_PREPARED_INST(double)
 #undef RANG_PREPARED_INST
 
+template int device_batch_vis2dirty<float>(BatchImage*, int, std::size_t, hipStream_t, std::string&);
+template int device_batch_vis2dirty<double>(BatchImage*, int, std::size_t, hipStream_t, std::string&);
+template int device_tune<float>(BatchImage&, const char*, int, double*, double*, int*, hipStream_t, std::string&);
+template int device_tune<double>(BatchImage&, const char*, int, double*, double*, int*, hipStream_t, std::string&);
 template int device_vis2dirty_fixed<float>(const Options&, const DevInputs&, const double*, float*,
                                            float*, double*, hipStream_t, std::string&);
 template int device_vis2dirty<float>(const Options&, const DevInputs&, const double*, float*,

@@ -278,13 +278,14 @@ double dw_of(double ofactor, double nm1min, double nm1max, double nshift) {
 // (wgridder.rs:876-926).
 bool cost_model(const Options& o, bool single, std::size_t nvis, double wmin_d, double wmax_d,
                 double nm1min, double nm1max, double nshift, int& kidx, std::size_t& nu,
-                std::size_t& nv, std::string& err) {
+                std::size_t& nv, std::string& err, std::vector<KernelChoice>* all = nullptr) {
     std::vector<int> cand;
     if (!candidates(o.epsilon, o.do_wgridding ? 3 : 2, single, o.sigma_min, o.sigma_max, cand, err))
         return false;
     constexpr double nref = 2048.0, costref = 0.0693;
     constexpr int vlen = 8;
     double mincost = 1e300;
+    if (all) all->clear();
     for (int i : cand) {
         const KernelRow& k = kRows[i];
         const int supp = k.w, nvec = (supp + vlen - 1) / vlen;
@@ -301,6 +302,7 @@ bool cost_model(const Options& o, bool single, std::size_t nvis, double wmin_d, 
             fft *= double(std::size_t((wmax_d - wmin_d) / dw + supp));
             grid *= supp;
         }
+        if (all) all->push_back(KernelChoice{i, supp, tnu, tnv, fft + grid});
         if (fft + grid < mincost) {
             mincost = fft + grid;
             kidx = i;
@@ -311,13 +313,8 @@ bool cost_model(const Options& o, bool single, std::size_t nvis, double wmin_d, 
     return true;
 }
 
-}  // namespace
-
-// Setup::new (wgridder.rs:604-754): n-1 extremes, nshift, kernel and grid,
-// and the w-plane range.
-bool make_geometry(const Options& o, bool single, std::size_t nvis, double wmin_d,
-                   double wmax_d, Geometry& g, std::string& err) {
-    g = Geometry();
+// The extremes of n - 1 over the image and the w shift (wgridder.rs:616-650).
+void nm1_range(const Options& o, double& nm1min, double& nm1max, double& nshift) {
     double r_l, r_m;
     recentring(o, r_l, r_m);
     const double px = o.pixsize_x, py = o.pixsize_y;
@@ -333,7 +330,8 @@ bool make_geometry(const Options& o, bool single, std::size_t nvis, double wmin_
         if (xmin * xmax < 0.0) xs.push_back(0.0);
         if (ymin * ymax < 0.0) ys.push_back(0.0);
     }
-    double nm1min = 1e300, nm1max = -1e300;
+    nm1min = 1e300;
+    nm1max = -1e300;
     for (double x : xs)
         for (double y : ys) {
             const double r2 = x * x + y * y;
@@ -342,7 +340,36 @@ bool make_geometry(const Options& o, bool single, std::size_t nvis, double wmin_
             nm1min = std::min(nm1min, n);
             nm1max = std::max(nm1max, n);
         }
-    const double nshift = (!o.allow_nshift || !o.do_wgridding) ? 0.0 : -0.5 * (nm1max + nm1min);
+    nshift = (!o.allow_nshift || !o.do_wgridding) ? 0.0 : -0.5 * (nm1max + nm1min);
+}
+
+}  // namespace
+
+bool kernel_choices(const Options& o, bool single, std::size_t nvis, double wmin_d, double wmax_d,
+                    std::vector<KernelChoice>& out, std::string& err) {
+    double nm1min, nm1max, nshift;
+    nm1_range(o, nm1min, nm1max, nshift);
+    int kidx = -1;
+    std::size_t nu = 0, nv = 0;
+    if (!cost_model(o, single, nvis, wmin_d, wmax_d, nm1min, nm1max, nshift, kidx, nu, nv, err, &out))
+        return false;
+    std::stable_sort(out.begin(), out.end(),
+                     [](const KernelChoice& a, const KernelChoice& b) { return a.cost < b.cost; });
+    return true;
+}
+
+// Setup::new (wgridder.rs:604-754): n-1 extremes, nshift, kernel and grid,
+// and the w-plane range.
+bool make_geometry(const Options& o, bool single, std::size_t nvis, double wmin_d,
+                   double wmax_d, Geometry& g, std::string& err) {
+    g = Geometry();
+    double r_l, r_m;
+    recentring(o, r_l, r_m);
+    const double px = o.pixsize_x, py = o.pixsize_y;
+    const double ls = o.flip_u ? -o.center_x : o.center_x;
+    const double ms = o.flip_v ? -o.center_y : o.center_y;
+    double nm1min, nm1max, nshift;
+    nm1_range(o, nm1min, nm1max, nshift);
     int kidx = -1;
     std::size_t nu = 0, nv = 0;
     if (o.kernel_index >= 0) {

@@ -79,4 +79,15 @@ bool check_freq(const double* freq, std::size_t nchan, std::string& err);
 bool make_geometry(const Options& o, bool single, std::size_t nvis, double wmin_d,
                    double wmax_d, Geometry& g, std::string& err);
 
+// The kernel rows the cost model weighs (one per support, the smallest
+// oversampling reaching epsilon), with their grids and modelled cost,
+// cheapest first.
+struct KernelChoice {
+    int index, support;
+    std::size_t nu, nv;
+    double cost;
+};
+bool kernel_choices(const Options& o, bool single, std::size_t nvis, double wmin_d, double wmax_d,
+                    std::vector<KernelChoice>& out, std::string& err);
+
 }  // namespace rang

@@ -46,6 +46,38 @@ class WgridderOptions(ctypes.Structure):
     ]
 
 
+class BatchImage(ctypes.Structure):
+    """include/rang_core_batch.h RangBatchImage."""
+
+    _fields_ = [
+        ("opts", WgridderOptions),
+        ("nrow", ctypes.c_size_t),
+        ("nchan", ctypes.c_size_t),
+        ("uvw", ctypes.c_void_p),
+        ("freq", ctypes.c_void_p),
+        ("vis", ctypes.c_void_p),
+        ("wgt", ctypes.c_void_p),
+        ("mask", ctypes.c_void_p),
+        ("dirty", ctypes.c_void_p),
+        ("vis_kind", ctypes.c_int32),
+        ("wgt_kind", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("info", ctypes.c_double * 14),
+    ]
+
+
+class TuneReport(ctypes.Structure):
+    """include/rang_core_batch.h RangTuneReport."""
+
+    _fields_ = [
+        ("ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("cached", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 # Every symbol of include/rang_core_wgridder.h with its ctypes signature.
 _vp, _sz, _i32, _dp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_double
 _OPT = ctypes.POINTER(WgridderOptions)
@@ -122,6 +154,16 @@ class GridOutputs(ctypes.Structure):
 GRID_N_STATS = 8
 GRID_N_TIMINGS = 14
 SIGNATURES = {
+    "rang_amd_wgridder_batch_vis2dirty_f32": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),
+    "rang_amd_wgridder_batch_vis2dirty_f64": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),
+    "rang_amd_wgridder_tune_f32": (
+        [ctypes.POINTER(BatchImage), ctypes.c_char_p, _i32, ctypes.POINTER(TuneReport), _vp],
+        _i32,
+    ),
+    "rang_amd_wgridder_tune_f64": (
+        [ctypes.POINTER(BatchImage), ctypes.c_char_p, _i32, ctypes.POINTER(TuneReport), _vp],
+        _i32,
+    ),
     "rang_core_wgridder_vis2dirty_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "rang_core_wgridder_vis2dirty_f32": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "rang_core_wgridder_dirty2vis_f64": ([_OPT, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i32),

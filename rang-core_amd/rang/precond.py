@@ -877,3 +877,83 @@ def safeguarded_fcg(measurement, precond, fallback, rhs, iterations, tolerance, 
     if stats is not None:
         stats["switched_at"] = switched_at
     return x, used, math.sqrt(dot(true_r, true_r)) / nb
+
+
+class SurrogateSolve:
+    """The preconditioner of the inner-outer solve: z ~ G~^{-1} r by conjugate
+    gradients on the surrogate G~ (the w-aware normal operator without
+    visibilities, rang.gram.GramOperator; rang-core's GramOperator::new,
+    core/src/gridder_gram.rs:226-407) preconditioned by `filt` (its circulant
+    inverse filter, :488-509), stopped at relative residual `tau` or after
+    `inner_max` steps. With `learned` (a FilteredPreconditioner) the inner
+    solve starts from the network's estimate instead of zero. Counts the
+    surrogate's and the network's applications."""
+
+    def __init__(self, surrogate, filt, *, tau=0.1, inner_max=400, learned=None):
+        self.G, self.M, self.tau, self.inner_max, self.learned = surrogate, filt, tau, inner_max, learned
+        self.surrogate_applies = 0
+        self.learned_applies = 0
+        self.inner_steps = []
+
+    def __call__(self, r, out=None):
+        lib, st = _lib(), _stream()
+        f64 = r.dtype == torch.float64
+        axpby = lib.rang_amd_nn_axpby_f64 if f64 else lib.rang_amd_nn_axpby_f32
+        idot = lib.rang_amd_nn_image_dot_f64 if f64 else lib.rang_amd_nn_image_dot_f32
+        n = r.numel()
+        acc = torch.empty(1, dtype=torch.float64, device=r.device)
+
+        def dot(a, b):
+            _ok(idot(_p(a), _p(b), _p(acc), 1, n, st))
+            return float(acc[0])
+
+        z = out if out is not None else torch.empty_like(r)
+        res = r.clone()
+        if self.learned is not None:
+            self.learned(r, out=z)
+            self.learned_applies += 1
+            q = self.G.apply(z)
+            self.surrogate_applies += 1
+            _ok(axpby(_p(q), _p(res), -1.0, 1.0, n, st))
+        else:
+            z.zero_()
+        n0 = math.sqrt(dot(r, r))
+        s = self.M.apply(res)
+        p = s.clone()
+        rs = dot(res, s)
+        k = 0
+        while k < self.inner_max and math.sqrt(dot(res, res)) > self.tau * n0:
+            q = self.G.apply(p)
+            self.surrogate_applies += 1
+            k += 1
+            pq = dot(p, q)
+            if not pq > 0.0:
+                break
+            a = rs / pq
+            _ok(axpby(_p(p), _p(z), a, 1.0, n, st))
+            _ok(axpby(_p(q), _p(res), -a, 1.0, n, st))
+            s = self.M.apply(res)
+            rs_n = dot(res, s)
+            _ok(axpby(_p(s), _p(p), 1.0, rs_n / rs, n, st))
+            rs = rs_n
+        self.inner_steps.append(k)
+        return z
+
+
+def inner_outer_solve(measurement, surrogate, filt, rhs, iterations, tolerance, *, tau=0.1, inner_max=400,
+                      learned=None, window=4, factor=0.5):
+    """Flexible CG on the exact normal operator A^H W A (every residual
+    judged by it), each step preconditioned by SurrogateSolve; when over
+    `window` outer steps the residual falls by less than `factor`, the rest
+    of the solve is plain CG (core/src/clean.rs:263-306), the fallback.
+    Returns (x, counts, rel): counts has the exact operator's applications
+    in the iteration ("exact"; one more forms the judged residual rel), the
+    surrogate's and the network's, the inner steps per outer step and the
+    step at which the fallback took over (None if it did not)."""
+    pre = SurrogateSolve(surrogate, filt, tau=tau, inner_max=inner_max, learned=learned)
+    st = {}
+    x, used, rel = safeguarded_fcg(measurement, pre, None, rhs, iterations, tolerance, window=window,
+                                   factor=factor, stats=st)
+    counts = {"exact": used, "surrogate": pre.surrogate_applies, "learned": pre.learned_applies,
+              "inner_steps": pre.inner_steps, "switched_at": st.get("switched_at")}
+    return x, counts, rel

@@ -106,7 +106,7 @@ def config1(seed=1234, ntime=100):
 def imaging_case_device(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, device, *, nant=64,
                         radius_m=8000.0, lat_deg=-30.7, dec_deg=-45.0, dt_s=8.0, nsrc=20,
                         sky_radius_deg=0.5, flux=(1e-3, 1.0), noise_jy=1e-3, seed=1234,
-                        rows_per_chunk=32768):
+                        rows_per_chunk=32768, freq=None):
     """The recipe of imaging_case with the visibilities formed on a GPU (for
     workloads whose direct sum is too large for the host): the same array,
     tracks and sky; the sum in float64 on the device, stored complex64; the
@@ -116,7 +116,8 @@ def imaging_case_device(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, device, *, n
 
     enu = antennas(nant, radius_m, seed)
     uvw = uvw_tracks(baselines_xyz(enu, lat_deg), ntime, dt_s, dec_deg)
-    freq = np.linspace(f_lo, f_hi, nchan)
+    freq = np.linspace(f_lo, f_hi, nchan) if freq is None else np.asarray(freq, np.float64)
+    nchan = freq.size
     flux_s, l_s, m_s = point_sky(nsrc, sky_radius_deg, flux[0], flux[1], seed + 7)
     d_uvw = torch.from_numpy(uvw).to(device)
     f = torch.from_numpy(freq / C_LIGHT).to(device)
@@ -153,3 +154,55 @@ def config2_device(device, seed=1234, ntime=900, nchan=256):
     case = imaging_case_device(ntime, nchan, 0.9e9, 1.7e9, 8192, 1.0, device, seed=seed)
     case["epsilon"] = 1e-5
     return case
+
+
+def flag_runs(ntime, nbl, nchan, fraction, rng, max_time=8):
+    """(ntime * nbl, nchan) bool flags: rectangles of contiguous time steps
+    (1..max_time, all baselines of those steps) by contiguous channels
+    (1..nchan/4) added until at least `fraction` of the cells is flagged."""
+    f = np.zeros((ntime, nchan), bool)
+    target = fraction * ntime * nchan
+    while f.sum() < target:
+        t0 = int(rng.integers(ntime))
+        c0 = int(rng.integers(nchan))
+        f[t0:t0 + int(rng.integers(1, max_time + 1)), c0:c0 + int(rng.integers(1, max(nchan // 4, 1) + 1))] = True
+    return np.repeat(f, nbl, axis=0)
+
+
+def config3_device(device, seed=1234, nimages=48, ntime=32, pix_arcsec=1.5, robust=0.0):
+    """Batched uneven sub-band imaging: `nimages` images with sides drawn from
+    512, 1024, 2048 and 4096, each with its own 8-128 channels (a contiguous
+    sub-band of 0.9-1.7 GHz at 0.78125 MHz spacing) over the same 2016
+    baselines x `ntime` times of 8 s, a flagged fraction uniform in 0-60 %
+    in contiguous time/channel runs, and float32 weights: the Briggs
+    (robust 0) imaging weight of each sample on the image's padded grid
+    (rang.core_weights, on the GPU) times its visibility weight (0 where
+    flagged, else 1). Returns a list of per-image cases (device tensors)."""
+    import torch
+
+    from . import core_weights
+
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([512, 1024, 2048, 4096], nimages)
+    df = 0.8e9 / 1024
+    cases = []
+    for k in range(nimages):
+        nchan = int(rng.integers(8, 129))
+        c0 = int(rng.integers(0, 1024 - nchan))
+        freq = 0.9e9 + (c0 + np.arange(nchan)) * df
+        c = imaging_case_device(ntime, nchan, 0.0, 0.0, int(sides[k]), pix_arcsec, device, seed=seed + 101 * k,
+                                freq=freq)
+        nbl = c["uvw"].shape[0] // ntime
+        frac = float(rng.uniform(0.0, 0.6))
+        flags = flag_runs(ntime, nbl, nchan, frac, rng)
+        vis_w = (~flags).astype(np.float32)
+        npad = core_weights.padded_size(int(sides[k]))
+        uvw_h = c["uvw"].cpu().numpy()
+        res = core_weights.imaging_weights(uvw_h, core_weights.wsclean_wavelengths(freq), vis_w,
+                                           np.ones(uvw_h.shape[0], bool), mode="briggs", robust=robust, width=npad,
+                                           height=npad, pixel_scale=c["pixsize"], return_grid=False)
+        c["wgt"] = torch.from_numpy(vis_w * res["weights"]).to(device)
+        c["flag_fraction"] = float(flags.mean())
+        c["side"] = int(sides[k])
+        cases.append(c)
+    return cases

@@ -12,7 +12,7 @@ from rang import _core, core_wgridder
 
 HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h",
            ROOT / "include" / "rang_core_precond.h", ROOT / "include" / "rang_core_gram.h",
-           ROOT / "include" / "rang_core_weights.h"]
+           ROOT / "include" / "rang_core_weights.h", ROOT / "include" / "rang_core_batch.h"]
 
 
 def declared():
@@ -22,7 +22,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 97
+    assert len(names) == 101
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
@@ -45,6 +45,25 @@ def test_options_struct_layout():
 
     assert ctypes.sizeof(_core.WgridderOptions) == 120
     assert _core.WgridderOptions.kernel_index.offset == 96
+
+
+def test_batch_structs_match_a_c_compiler(tmp_path):
+    # RangBatchImage / RangTuneReport as gcc lays them out from the header
+    import ctypes
+    import subprocess
+
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "rang_core_batch.h"\n'
+        "int main(void) { printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(RangBatchImage),"
+        " offsetof(RangBatchImage, nrow), offsetof(RangBatchImage, vis_kind), offsetof(RangBatchImage, tile),"
+        " offsetof(RangBatchImage, info), sizeof(RangTuneReport)); return 0; }\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    B = _core.BatchImage
+    assert got == [ctypes.sizeof(B), B.nrow.offset, B.vis_kind.offset, B.tile.offset, B.info.offset,
+                   ctypes.sizeof(_core.TuneReport)]
 
 
 def test_kernel_rows_through_the_abi_are_the_table():
