@@ -16,10 +16,14 @@ The JSON line also carries: the roofline of the step's dominant kernel
 stream, against 8 TB/s); the oracle (the CPU restatement of the reference
 gridder, OpenMP) timed on this host's cores on a bounded sample of the same
 workload and scaled to it; the same measurements on the small workload
-(3,225,600 visibilities, 1024 x 1024 at 2 arcsec) under "config_small"; and,
-under "gram_solve", the Gram applications plain CG and the learned
-preconditioner need to reach a relative residual of 1e-3 on a reduced
-certified-imaging case (--no-gram skips it); under "published", the
+(3,225,600 visibilities, 1024 x 1024 at 2 arcsec) under "config_small";
+under "sub_band_batch", 48 uneven sub-band images in batch calls (one
+gridding launch per call) on tuned plans at epsilon 1e-5 and 1e-4
+(--no-batch skips it); under "gram_solve", the exact Gram applications and
+seconds that plain CG, the inverse-filter PCG and the inner-outer solve (with
+and without the learned warm start) need to reach a relative residual of
+1e-4 on a 2048^2 image through 19.4 M visibilities (--no-gram skips it);
+under "published", the
 exact gridder and the w-stacking gridder on a 23.8 M-visibility 1024^2 case
 beside the reference's published timings for that size (--no-published
 skips it).
@@ -530,52 +534,198 @@ def published(dev):
     }
 
 
-def gram_solve(dev, tol=1e-3, budget=2000, steps=400):
-    """The Gram-solve half of the metric on a reduced certified-imaging case
-    (2016 baselines x 32 times x 8 channels, 128^2 at 2 arcsec, float64,
-    epsilon 1e-7): Gram applications to a relative residual `tol` for plain
-    CG and for flexible CG with the learned preconditioner (trained on this
-    plan's own residuals; its training applications reported beside), each
-    result's residual recomputed with the exact Gram operator."""
+def run_batch(dev, steps=3, ntime=100, eps_list=(1e-5, 1e-4), parity_images=3, dft_images=3):
+    """Batched uneven sub-band imaging: 48 images of 512^2-4096^2 at 1.5
+    arcsec, each with its own 8-128 channels of 0.9-1.7 GHz over 2016
+    baselines x `ntime` times, 0-60 % flagged in contiguous runs, float32
+    weights = Briggs (robust 0) imaging weights x visibility weights. Per
+    epsilon: one batch call on the cost model's plans, one call per image,
+    the tuner over every image (uncached: a fresh cache file), and the batch
+    call on the tuned plans; value = input samples / wall time of a batch
+    call. Errors: the `parity_images` images with the fewest samples against
+    the oracle at the tuned plan, and `dft_images` images on 256 pixels
+    against the FP64 direct sum."""
+    import tempfile
+
+    import numpy as np
     import torch
 
-    from rang import precond as P
-    from rang import synth
-    from rang.measurement import WgridMeasurement, conjugate_gradients
+    sys.path.insert(0, str(ROOT / "tests"))
+    from direct import dirty_pixels, errors
 
-    c = synth.imaging_case_device(32, 8, 1.30e9, 1.45e9, 128, 2.0, dev)
-    m = WgridMeasurement(c["uvw"], c["freq"], c["vis"], c["wgt"].double(), 128, c["pixsize"], 1e-7)
-    rhs, _ = m.residual(torch.zeros((128, 128), dtype=torch.float64, device=dev))
+    from oracle import oracle as O
+    from rang import batch, synth
+    from rang import device as D
+
+    cases = synth.config3_device(dev, ntime=ntime)
+    nsamp = sum(int(c["vis"].numel()) for c in cases)
+    progress(f"sub-band batch formed: {len(cases)} images, {nsamp} samples")
+
+    def opts(c, eps):
+        return [c["side"], c["side"], c["pixsize"], c["pixsize"], eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    out = {
+        "workload": f"{len(cases)} images, sides {{512, 1024, 2048, 4096}} at 1.5 arcsec, 8-128 channels each "
+                    f"(0.78125 MHz spacing in 0.9-1.7 GHz), 2016 baselines x {ntime} times, 0-60 % flagged in "
+                    "contiguous runs, Briggs robust 0 x visibility weights (float32), single precision",
+        "images": len(cases),
+        "sides": sorted(int(c["side"]) for c in cases),
+        "samples": nsamp,
+        "unit": "Mvis/s (input samples, flagged included)",
+        "by_epsilon": {},
+    }
+    order = sorted(range(len(cases)), key=lambda k: cases[k]["vis"].numel())
+    for eps in eps_list:
+        images = [batch.Image(opts(c, eps), c["uvw"], c["freq"], c["vis"], c["wgt"]) for c in cases]
+        outs = [torch.empty((c["side"], c["side"]), dtype=torch.float32, device=dev) for c in cases]
+        t_def = timed(lambda: batch.vis2dirty(images, single=True, out=outs))
+        active = int(sum(im.info[5] for im in images))
+
+        def one_by_one():
+            for im, o, c in zip(images, outs, cases):
+                D.vis2dirty(im.opts, im.uvw, c["freq"], im.vis, im.wgt, single=True, out=(o, None))
+
+        t_one = timed(one_by_one)
+        with tempfile.TemporaryDirectory() as tmp:
+            t0 = time.perf_counter()
+            reps = [batch.tune(im, single=True, cache_path=os.path.join(tmp, "tuning.txt"), kernels=3)
+                    for im in images]
+            t_tune = time.perf_counter() - t0
+        t_tuned = timed(lambda: batch.vis2dirty(images, single=True, out=outs))
+        torch.cuda.synchronize()
+        errs = []
+        for k in order[:parity_images]:
+            c, im = cases[k], images[k]
+            h = {key: c[key].cpu().numpy() for key in ("uvw", "freq", "vis", "wgt")}
+            ref, _ = O.vis2dirty(h["uvw"], h["freq"], h["vis"], h["wgt"], npix_x=c["side"], npix_y=c["side"],
+                                 pixsize_x=c["pixsize"], pixsize_y=c["pixsize"], epsilon=eps, single=True,
+                                 kernel=(int(im.info[0]), int(im.info[2]), int(im.info[3])))
+            e = errors(outs[k].double().cpu().numpy(), ref["dirty"])
+            errs.append({"image": k, "side": int(c["side"]), "samples": int(c["vis"].numel()),
+                         "vs_oracle": {"l2": e[0], "linf_over_peak": e[1], "sample": "every pixel"}})
+        for k in order[-dft_images:]:
+            c = cases[k]
+            n, pix = int(c["side"]), c["pixsize"]
+            g = torch.Generator(device=dev).manual_seed(k)
+            i = torch.randint(0, n, (256,), device=dev, generator=g)
+            j = torch.randint(0, n, (256,), device=dev, generator=g)
+            ref = dirty_pixels(c["uvw"], c["freq"], c["vis"], c["wgt"], i, j, n, n, pix, pix)
+            e = errors(outs[k][i, j].double().cpu().numpy(), ref.cpu().numpy())
+            errs.append({"image": k, "side": n, "samples": int(c["vis"].numel()),
+                         "vs_fp64_dft": {"l2": e[0], "linf_over_peak": e[1], "sample": "256 pixels"}})
+        out["by_epsilon"][f"{eps:g}"] = {
+            "value": round(nsamp / t_tuned / 1e6, 2),
+            "ms_per_batch": round(t_tuned * 1e3, 3),
+            "active_visibilities": active,
+            "untuned_batch": {"value": round(nsamp / t_def / 1e6, 2), "ms": round(t_def * 1e3, 3)},
+            "one_call_per_image": {"value": round(nsamp / t_one / 1e6, 2), "ms": round(t_one * 1e3, 3)},
+            "tuning_seconds": round(t_tune, 2),
+            "tuned_choices": [{"kernel_index": int(im.opts[15]), "tile": int(im.tile),
+                               "model_ms": round(r["default_ms"], 3), "tuned_ms": round(r["ms"], 3)}
+                              for im, r in zip(images, reps)],
+            "errors": errs,
+        }
+        progress(f"sub-band batch eps {eps:g}: {nsamp / t_tuned / 1e6:.1f} Mvis/s tuned, "
+                 f"{nsamp / t_def / 1e6:.1f} untuned, {nsamp / t_one / 1e6:.1f} one call per image")
+    return out
+
+
+def gram_solve(dev, tol=1e-4, budget=4000, train_steps=60, tau=0.1):
+    """The Gram-solve half of the metric at the learned-preconditioner
+    workload: a 2048^2 image at 1.5 arcsec through 2016 baselines x 300
+    times x 32 channels (19,353,600 visibilities, 1.30-1.45 GHz, unit
+    weights), eps 1e-5, single precision; A^H W A x = b, b = A^H W A x_sky
+    of a 300-point sky (log-uniform 0.1 mJy - 1 Jy), solved from zero to
+    relative residual `tol` judged by the exact operator. Exact-operator
+    applications and seconds for: plain CG; CG preconditioned by the
+    circulant inverse filter; the inner-outer solve (flexible CG on the
+    exact operator, each step a PCG solve to `tau` on the w-aware Toeplitz
+    surrogate, which costs no gridding); and the same with the learned
+    warm start (U-Net 4 levels x 32 base channels, BF16 on MFMA, trained
+    `train_steps` steps of 4 skies on the surrogate's images and on residuals
+    of the filter-preconditioned solve)."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from rang import gram, synth
+    from rang import precond as P
+    from rang.measurement import WgridMeasurement
+
+    npix, ntime, nchan = 2048, 300, 32
+    enu = synth.antennas(64, 8000.0, 1234)
+    uvw = synth.uvw_tracks(synth.baselines_xyz(enu, -30.7), ntime, 8.0, -45.0)
+    freq = np.linspace(1.30e9, 1.45e9, nchan)
+    tu, tf = torch.from_numpy(uvw).to(dev), torch.from_numpy(freq).to(dev)
+    vis = torch.zeros((uvw.shape[0], nchan), dtype=torch.complex64, device=dev)
+    wgt = torch.ones((uvw.shape[0], nchan), dtype=torch.float32, device=dev)
+    pix, eps = 1.5 * synth.ARCSEC, 1e-5
     t0 = time.perf_counter()
-    _, used_cg, rel_cg = conjugate_gradients(m, rhs, budget, tol)
+    m = WgridMeasurement(tu, tf, vis, wgt, npix, pix, eps, single=True)
+    o = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    psf = gram.GramOperator.no_w(o, tu, tf, m.wgt, eps, single=True)
+    filt = psf.inverse_filter(1e-3 * psf.stats()["spectral_peak"])
+    G = gram.GramOperator.new(o, tu, tf, m.wgt, gram.WExpansion(8, 1.0, 1e-13), eps, single=True)
     torch.cuda.synchronize()
-    t_cg = time.perf_counter() - t0
-    net = P.UNet(128, 128, batch=8, levels=3, base=32, seed=2)
-    tr = P.Trainer(m, net, seed=11)
-    t0 = time.perf_counter()
-    tr.collect(rhs_count=16, iterations=200)
-    tr.train(steps, lr=2e-3, lr_end=1e-4)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    M = P.Preconditioner(m, net)
-    t0 = time.perf_counter()
-    _, used_p, rel_p = P.preconditioned_cg(m, M, rhs, budget, tol)
-    torch.cuda.synchronize()
-    t_p = time.perf_counter() - t0
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(1)
+    xs = torch.zeros((npix, npix), dtype=torch.float32, device=dev)
+    k = 300
+    ii = torch.from_numpy(rng.integers(npix // 8, 7 * npix // 8, k)).to(dev)
+    jj = torch.from_numpy(rng.integers(npix // 8, 7 * npix // 8, k)).to(dev)
+    xs[ii, jj] = torch.from_numpy(np.exp(rng.uniform(np.log(1e-4), 0, k)).astype(np.float32)).to(dev)
+    b = m.plan.gram(xs).clone()
+
+    def judged(x):
+        r = b.double() - m.plan.gram(x.contiguous()).double()
+        return math.sqrt(float((r * r).sum()) / float((b.double() * b.double()).sum()))
+
+    def clock(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = fn()
+        torch.cuda.synchronize()
+        return res, round(time.perf_counter() - t, 3)
+
+    (x_cg, n_cg, _), t_cg = clock(lambda: m.plan.conjugate_gradients(b, budget, tol))
+    progress(f"Gram solve: CG {n_cg} applications, {t_cg} s")
+    (x_p, n_p, _), t_p = clock(lambda: m.plan.preconditioned_cg(filt, b, budget, tol))
+    (x_io, c_io, _), t_io = clock(lambda: P.inner_outer_solve(m, G, filt, b, 200, tol, tau=tau))
+    net = P.UNet(npix, npix, batch=4, levels=4, base=32, seed=1)
+    tr = P.FilteredTrainer(m, net, filt, G)
+    _, t_train = clock(lambda: (tr.collect(rhs_count=2, iterations=40, keep=(0, 2, 5, 10, 20, 39)),
+                                tr.train(train_steps, lr=1e-3, lr_end=1e-4)))
+    learned = P.FilteredPreconditioner(m, net, filt)
+    (x_l, c_l, _), t_l = clock(lambda: P.inner_outer_solve(m, G, filt, b, 200, tol, tau=tau, learned=learned))
     return {
-        "workload": "2016 baselines x 32 times x 8 channels (516,096 vis), 128x128 at 2 arcsec, "
-                    "eps 1e-7, float64; A^H W A x = A^H W v from zero",
+        "workload": "2048x2048 at 1.5 arcsec, 2016 baselines x 300 times x 32 channels (19,353,600 vis, "
+                    "1.30-1.45 GHz), eps 1e-5, single precision; A^H W A x = A^H W A x_sky (300 points) from zero",
         "tolerance": tol,
-        "budget": budget,
-        "cg_applies": used_cg,
-        "cg_relative_residual": rel_cg,
-        "cg_seconds": round(t_cg, 3),
-        "learned_applies": used_p,
-        "learned_exact_relative_residual": rel_p,
-        "learned_seconds": round(t_p, 3),
-        "training_gram_applies": tr.gram_applies,
-        "training_seconds": round(t_train, 2),
-        "network": f"U-Net 3 levels x 32 base channels, BF16 on MFMA, {net.parameters()} weights",
+        "judge": "relative residual |b - A^H W A x| / |b| with the exact operator, recomputed after each solve",
+        "setup_seconds": round(t_setup, 2),
+        "cg": {"exact_applies": n_cg, "seconds": t_cg, "relative_residual": judged(x_cg)},
+        "pcg_inverse_filter": {"exact_applies": n_p, "seconds": t_p, "relative_residual": judged(x_p)},
+        "inner_outer": {"exact_applies": c_io["exact"], "surrogate_applies": c_io["surrogate"],
+                        "seconds": t_io, "relative_residual": judged(x_io), "tau": tau,
+                        "fallback_at": c_io["switched_at"]},
+        "inner_outer_learned": {"exact_applies": c_l["exact"], "surrogate_applies": c_l["surrogate"],
+                                "network_applies": c_l["learned"], "seconds": t_l,
+                                "relative_residual": judged(x_l), "fallback_at": c_l["switched_at"],
+                                "training_seconds": t_train, "training_exact_applies": tr.exact_applies,
+                                "training_surrogate_applies": tr.gram_applies,
+                                "training_skies": train_steps * net.batch // 2 + 2},
+        "network": f"U-Net 4 levels x 32 base channels, BF16 on MFMA, {net.parameters()} weights",
+        "exact_over_surrogate_cost": "one surrogate application costs about 0.3 of an exact one",
     }
 
 
@@ -588,10 +738,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the small workload")
     ap.add_argument("--no-gram", action="store_true", help="skip the Gram-solve comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the sub-band batch")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the cases the reference published timings for")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
+    ap.add_argument("--profile", choices=("large", "small"),
+                    help="only run --warmup + --steps full vis2dirty + dirty2vis calls of one workload and print "
+                         "their stage times (for profiler runs; no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run a plain copy of 1 GiB (8 launches), the reference for HBM "
                          "byte counters")
@@ -617,6 +771,20 @@ def main():
 
     dev = torch.device("cuda:0")
     from rang import device as D
+
+    if args.profile:
+        if args.profile == "large":
+            case = synth.config2_device(dev)
+        else:
+            small = synth.config1()
+            case = dict(small)
+            for k in ("uvw", "freq", "vis", "wgt"):
+                case[k] = torch.from_numpy(small[k]).to(dev)
+        progress("visibilities formed")
+        el, info, means = run_gpu(case, args.steps, args.warmup, verbose=True)
+        print(json.dumps({"profile": args.profile, "seconds": round(el, 4), "steps": args.steps,
+                          "stages_ms": {f"{d}:{n}": round(v, 4) for (d, n), v in sorted(means.items())}}))
+        return
 
     if not args.only_small:
         big = synth.config2_device(dev)
@@ -644,6 +812,7 @@ def main():
         f64 = run_f64(big, 2)
         progress(f"large workload, f64: {f64['value']} Mvis/s")
         del big
+        D.release()
         torch.cuda.empty_cache()
 
     small_line = None
@@ -690,10 +859,16 @@ def main():
             return
     cpu = None if args.no_cpu_baseline else cpu_large(info, nvis_total)
     pub = None if args.no_published else published(dev)
+    sub = None
+    if not args.no_batch:
+        sub = run_batch(dev)
+        D.release()
+        torch.cuda.empty_cache()
     gram = None
     if not args.no_gram:
         gram = gram_solve(dev)
-        progress(f"Gram solve: {json.dumps({k: gram[k] for k in gram if k.endswith('applies')})}")
+        progress("Gram solve: " + json.dumps({k: gram[k].get("exact_applies") for k in
+                                              ("cg", "pcg_inverse_filter", "inner_outer", "inner_outer_learned")}))
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -729,6 +904,7 @@ def main():
         "per_call": per_call,
         "cpu_baseline": cpu,
         "config_small": small_line,
+        "sub_band_batch": sub,
         "gram_solve": gram,
         "published": pub,
     }

@@ -341,6 +341,13 @@ size_t rang_amd_wgridder_prepared_memory(const RangPrepared *p);
 void rang_amd_wgridder_prepared_release_grids(RangPrepared *p);
 void rang_amd_wgridder_prepared_free(RangPrepared *p);
 
+/* No counterpart in the reference: frees the device memory the library
+ * keeps between calls on every device (its per-device workspaces and
+ * sub-band batch pools; not the fixed block of the allocation-free entry
+ * point, nor the buffers of live prepared plans). Waits for the devices'
+ * work first; the next call allocates again. */
+int32_t rang_amd_release_workspaces(void);
+
 /* Per-stage timing (HIP events on the call's stream) of this thread's later
  * calls: on != 0 enables it; count, names and milliseconds of the last call. */
 void rang_amd_stage_timing(int32_t on);

@@ -966,6 +966,15 @@ int32_t rang_amd_wgridder_tune_f64(RangBatchImage* image, const char* cache_path
     return tune_abi<double>(image, cache_path, kernels, report, stream);
 }
 
+int32_t rang_amd_release_workspaces(void) {
+    try {
+        release_workspaces();
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(2, std::string("failure in the gridder: ") + e.what());
+    }
+}
+
 int32_t rang_amd_stage_count(void) { return stage_timing_count(); }
 const char* rang_amd_stage_name(int32_t i) { return stage_timing_name(i); }
 double rang_amd_stage_ms(int32_t i) { return stage_timing_ms(i); }

@@ -128,6 +128,9 @@ template <class T>
 int prepared_power(PreparedBase* p, std::uint64_t seed, double tolerance, std::uint64_t max_iterations,
                    double* value, std::uint64_t* used, hipStream_t stream, std::string& err);
 
+// Frees the device buffers kept between calls (workspaces, batch pools).
+void release_workspaces();
+
 // Per-kernel timing of the last call on this thread (hipEvents on the call's
 // stream; enabled by set_stage_timing). Names and milliseconds.
 void set_stage_timing(bool on);

@@ -276,13 +276,34 @@ Workspace& scratch_workspace(int device) {
     return *w;
 }
 
+std::mutex g_ws_mu;
+std::map<int, std::unique_ptr<Workspace>> g_ws_all;
+
 Workspace& workspace(int device) {
-    static std::mutex mu;
-    static std::map<int, std::unique_ptr<Workspace>> all;
-    std::lock_guard<std::mutex> lk(mu);
-    auto& w = all[device];
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto& w = g_ws_all[device];
     if (!w) w.reset(new Workspace());
     return *w;
+}
+
+// Frees a (non-arena) workspace's device and pinned buffers and forgets its
+// cached tables; its generation moves on, so graphs holding the old
+// addresses are dropped before their next replay. The caller holds ws.mu.
+void free_workspace(Workspace& ws) {
+    for (int k = 0; k < kSlotCount; ++k) {
+        if (ws.ptr[k] && !ws.arena) (void)hipFree(ws.ptr[k]);
+        ws.ptr[k] = nullptr;
+        ws.size[k] = 0;
+    }
+    if (ws.pinned) (void)hipHostFree(ws.pinned);
+    ws.pinned = nullptr;
+    ws.pinned_size = 0;
+    ws.table_key.clear();
+    ws.foc_key.clear();
+    ws.order_key[0].clear();
+    ws.order_key[1].clear();
+    ws.long_key.clear();
+    ++ws.gen;
 }
 
 // ------------------------------------------------------------ stage timing
@@ -344,6 +365,8 @@ struct Geo {
     // Transforms longer than one LDS along u or v: n = n1 n2 (0: in LDS).
     // Their forward results sit at (k mod n1) n2 + k / n1.
     int lu1, lu2, lv1, lv2;
+    // Gridding on the matrix cores (16 x 16 tiles, k_grid_mfma).
+    int mfma;
 };
 
 __device__ __forceinline__ int upos(const Geo& g, int u) { return g.lu1 ? (u % g.lu1) * g.lu2 + u / g.lu1 : u; }
@@ -1379,6 +1402,187 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid_jobs(const GridJob<T
 #undef JOB_CASE
 }
 
+// ------------------------------------------------------------ gridding on the matrix cores
+
+// Single-precision w-gridding on 16 x 16 tiles with the matrix cores: one
+// wave per item (tile, group of up to kMfmaPlanes planes, chunk), whose
+// accumulators are the item's planes of the tile held in the MFMA result
+// registers, so that no LDS atomics are needed. Four visibilities at a
+// time: with A[i][k] = ku_k(i - ou_k) kw_k(plane) (lane 16 k + i) and
+// B[k][j] = kv_k(j - ov_k) v_k (lane 16 k + j), one
+// v_mfma_f32_16x16x4_f32 per plane and part adds sum_k A[i][k] B[k][j] to
+// cell (i, j) of the plane, which is the gridding sum (wgridder.rs:2098-2156)
+// for those four visibilities; taps outside a visibility's footprint or
+// planes are zero. Lane 16 k + t evaluates tap t of visibility k at its u,
+// v and w arguments once; the operands fetch theirs across lanes. The
+// result register r of lane l holds cell (4 (l / 16) + r, l mod 16).
+constexpr int kMfmaPlanes = 8;
+constexpr int kMfmaTile = 16;
+typedef float mfma_v4 __attribute__((ext_vector_type(4)));
+
+template <int S>
+struct MfmaShared {
+    float coef[2 * ne_of<S>() * S];
+    std::uint32_t start[9], pre[10];
+    Rec<float> rec[64];
+};
+
+template <int S, bool FORM>
+__device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restrict__ recs,
+                                               const std::uint32_t* __restrict__ off,
+                                               const float* __restrict__ coeff, DevIn in, int G, int ngroups,
+                                               std::uint32_t rc, const std::uint32_t* __restrict__ item_off,
+                                               const std::uint32_t* __restrict__ order,
+                                               cx<float>* __restrict__ grid, std::uint32_t item, float* coef,
+                                               std::uint32_t* s_start, std::uint32_t* s_pre, Rec<float>* s_rec) {
+    constexpr int NE = ne_of<S>();
+    const int lane = threadIdx.x;
+    const int ntg = g.ntu * g.ntv * ngroups;
+    int tg;
+    std::uint32_t chunk, nchunks;
+    if (!find_item(item_off, ntg, item, tg, chunk, nchunks)) return;
+    const int rank = tg / ngroups, grp = tg - rank * ngroups;
+    const int tile = int(order[rank]);
+    const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
+    const int P = g.nplanes;
+    const int p0 = grp * G, p1 = min(P, p0 + G), np = p1 - p0;
+    for (int e = lane; e < 2 * NE * S; e += 64) coef[e] = coeff[e];
+    const int mlo = max(0, p0 - S + 1), mhi = min(g.nb - 1, p1 - 1);
+    if (lane < 9) {
+        std::uint32_t a, b;
+        source_range(g, off, tu, tv, lane, mlo, mhi, a, b);
+        s_start[lane] = a;
+        s_pre[lane + 1] = b - a;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_pre[0] = 0;
+        for (int k = 1; k <= 9; ++k) s_pre[k] += s_pre[k - 1];
+    }
+    __syncthreads();
+    const std::uint32_t jbeg = chunk * rc, jend = min(s_pre[9], jbeg + rc);
+    const int k = lane >> 4, i = lane & 15;
+    TapReg<float, S> tr;
+    tr.load(coef, min(i, S - 1));
+    const cx<float>* stored = static_cast<const cx<float>*>(in.stored);
+    mfma_v4 are[kMfmaPlanes], aim[kMfmaPlanes];
+#pragma unroll
+    for (int q = 0; q < kMfmaPlanes; ++q) {
+        are[q] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+        aim[q] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+    }
+    for (std::uint32_t base = jbeg; base < jend; base += 64) {
+        const int nrec = int(min(jend - base, 64u));
+        Rec<float> r{};
+        if (lane < nrec) {
+            r = grid_fetch(g, recs, s_start, s_pre, base + lane, p0, p1, stored);
+            if constexpr (FORM) {
+                const float fl = (r.idx & kFlip) ? -1.f : 1.f;
+                const cx<float> d = cmul(vis_get<float>(in, r.idx & ~kFlip), cx<float>{r.re, r.im * fl});
+                r.re = d.re;
+                r.im = d.im * fl;
+            }
+        } else {
+            // no taps anywhere: offsets far outside the tile, an empty plane span
+            r.pos = 0u;
+            r.re = r.im = 0.f;
+        }
+        __syncthreads();  // the previous batch is consumed
+        s_rec[lane] = r;
+        __syncthreads();
+        for (int g4 = 0; g4 < nrec; g4 += 4) {
+            const Rec<float>& rk = s_rec[g4 + k];
+            const std::uint32_t pos = rk.pos;
+            const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
+            int tlo, thi, q0, mp;
+            taps_on_item(pos, p0, tlo, thi, q0, mp);
+            const float tU = tr.eval(rk.x0), tV = tr.eval(rk.y0), tW = tr.eval(warg_of(rk));
+            const int au = i - ou, av = i - ov;
+            const bool inu = au >= 0 && au < S, inv = av >= 0 && av < S;
+            const float ku = __shfl(tU, 16 * k + (inu ? au : 0), 64);
+            const float kv = __shfl(tV, 16 * k + (inv ? av : 0), 64);
+            const float A0 = inu ? ku : 0.f;
+            const float Bk = inv ? kv : 0.f;
+            const float bre = Bk * rk.re, bim = Bk * rk.im;
+            // the group's planes: the union of its visibilities' item planes
+            const int qa = thi > tlo ? q0 : kMfmaPlanes, qb = thi > tlo ? q0 + thi - tlo : 0;
+            int qlo = min(min(__builtin_amdgcn_readlane(qa, 0), __builtin_amdgcn_readlane(qa, 16)),
+                          min(__builtin_amdgcn_readlane(qa, 32), __builtin_amdgcn_readlane(qa, 48)));
+            int qhi = max(max(__builtin_amdgcn_readlane(qb, 0), __builtin_amdgcn_readlane(qb, 16)),
+                          max(__builtin_amdgcn_readlane(qb, 32), __builtin_amdgcn_readlane(qb, 48)));
+            qhi = min(qhi, np);
+#pragma unroll
+            for (int q = 0; q < kMfmaPlanes; ++q) {
+                if (q < qlo || q >= qhi) continue;
+                const int t = q - q0 + tlo;
+                const bool inw = t >= tlo && t < thi;
+                const float kw = __shfl(tW, 16 * k + (inw ? t : 0), 64);
+                const float a = inw ? A0 * kw : 0.f;
+                are[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, are[q], 0, 0, 0);
+                aim[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, aim[q], 0, 0, 0);
+            }
+        }
+    }
+    // flush: register r of lane l is cell (4 (l / 16) + r, l mod 16)
+    const long long plane = (long long)g.nu * g.nv;
+    int gcol = tv * kMfmaTile + i + g.nv / 2;
+    if (gcol >= g.nv) gcol -= g.nv;
+#pragma unroll
+    for (int q = 0; q < kMfmaPlanes; ++q) {
+        if (q >= np) break;
+        cx<float>* dst = grid + (long long)(p0 + q) * plane + gcol;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            int grow = tu * kMfmaTile + 4 * k + rr + g.nu / 2;
+            if (grow >= g.nu) grow -= g.nu;
+            const float vre = are[q][rr], vim = aim[q][rr];
+            cx<float>* c = dst + (long long)grow * g.nv;
+            if (nchunks == 1) {
+                *c = cx<float>{vre, vim};
+            } else if (vre != 0.f || vim != 0.f) {
+                atomicAdd(&c->re, vre);
+                atomicAdd(&c->im, vim);
+            }
+        }
+    }
+}
+
+template <int S, bool FORM>
+__global__ __launch_bounds__(64) void k_grid_mfma(Geo g, const Rec<float>* __restrict__ recs,
+                                                  const std::uint32_t* __restrict__ off,
+                                                  const float* __restrict__ coeff, DevIn in, int G,
+                                                  int ngroups, std::uint32_t rc,
+                                                  const std::uint32_t* __restrict__ item_off,
+                                                  const std::uint32_t* __restrict__ order,
+                                                  cx<float>* __restrict__ grid) {
+    __shared__ MfmaShared<S> sh;
+    grid_item_mfma<S, FORM>(g, recs, off, coeff, in, G, ngroups, rc, item_off, order, grid, blockIdx.x, sh.coef,
+                            sh.start, sh.pre, sh.rec);
+}
+
+// Every job's items in one launch on the matrix cores (jobs as k_grid_jobs,
+// each gridded by k_grid_mfma's item body at its support).
+template <bool FORM>
+__global__ __launch_bounds__(64) void k_grid_mfma_jobs(const GridJob<float>* __restrict__ jobs, int njobs) {
+    __shared__ MfmaShared<8> sh;
+    int j = 0;
+    for (int k = 1; k < njobs; ++k)
+        if (jobs[k].first <= blockIdx.x) j = k;
+    const GridJob<float>& J = jobs[j];
+    const std::uint32_t item = blockIdx.x - J.first;
+    switch (J.g.supp) {
+#define MJOB_CASE(SS)                                                                                    \
+    case SS:                                                                                             \
+        grid_item_mfma<SS, FORM>(J.g, J.recs, J.off, J.coeff, J.in, J.G, J.ngroups, J.rc, J.item_off,    \
+                                 J.order, J.grid, item, sh.coef, sh.start, sh.pre, sh.rec);              \
+        break;
+        MJOB_CASE(4) MJOB_CASE(5) MJOB_CASE(6) MJOB_CASE(7) MJOB_CASE(8)
+#undef MJOB_CASE
+    default:
+        break;
+    }
+}
+
 // ------------------------------------------------------------ transforms
 
 // Largest transform held in LDS: 128 KiB of complex values.
@@ -2220,6 +2424,28 @@ void launch_grid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo&
     HIPCHK(hipGetLastError());
 }
 
+void launch_grid_mfma(int S, int nblocks, bool form, hipStream_t st, const Geo& g, const Rec<float>* recs,
+                      const std::uint32_t* off, const float* coeff, const DevIn& in, int G, int ngroups,
+                      std::uint32_t rc, const std::uint32_t* item_off, const std::uint32_t* order,
+                      cx<float>* out) {
+#define MFMA_CASE(SS)                                                                                   \
+    case SS:                                                                                            \
+        if (form)                                                                                       \
+            k_grid_mfma<SS, true><<<nblocks, 64, 0, st>>>(g, recs, off, coeff, in, G, ngroups, rc, item_off, \
+                                                          order, out);                                  \
+        else                                                                                            \
+            k_grid_mfma<SS, false><<<nblocks, 64, 0, st>>>(g, recs, off, coeff, in, G, ngroups, rc, item_off, \
+                                                           order, out);                                 \
+        break;
+    switch (S) {
+        MFMA_CASE(4) MFMA_CASE(5) MFMA_CASE(6) MFMA_CASE(7) MFMA_CASE(8)
+    default:
+        throw DevError{"unsupported kernel support"};
+    }
+#undef MFMA_CASE
+    HIPCHK(hipGetLastError());
+}
+
 template <class T, bool STAGE>
 void launch_degrid(int S, int nblocks, std::size_t lds, hipStream_t st, const Geo& g,
                    const Rec<T>* recs, const std::uint32_t* off, const T* coeff,
@@ -2514,6 +2740,15 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
+// Gridding on the matrix cores unless RANG_GRID_MFMA is 0.
+bool mfma_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("RANG_GRID_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <class T>
 bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err, int tile = 0) {
     g.nx = int(G.nx);
@@ -2555,8 +2790,14 @@ bool fill_geo(const Geometry& G, Geo& g, int nout, bool degrid, std::string& err
     } else if (tile > 0) {
         target = std::min(tile, hi);  // the tuner's tile side
     }
-    g.Tu = choose_tile(g.nu, g.supp, std::min(hi, 64), target);
-    g.Tv = choose_tile(g.nv, g.supp, std::min(hi, 64), target);
+    // single-precision w-gridding of one image on 16 x 16 tiles on the
+    // matrix cores, unless a tile side is asked for
+    g.mfma = (sizeof(T) == 4 && !degrid && nout == 1 && g.wgrid && g.supp <= 8 && tile == 0 &&
+              g.nu % kMfmaTile == 0 && g.nv % kMfmaTile == 0 && mfma_enabled())
+                 ? 1
+                 : 0;
+    g.Tu = g.mfma ? kMfmaTile : choose_tile(g.nu, g.supp, std::min(hi, 64), target);
+    g.Tv = g.mfma ? kMfmaTile : choose_tile(g.nv, g.supp, std::min(hi, 64), target);
     if (g.Tu < 0 || g.Tv < 0) {
         err = "no tile size divides the grid";
         return false;
@@ -3109,13 +3350,19 @@ void grid_prepare(Workspace& ws, const Geometry& G, const Geo& g, const std::uin
     gp.grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P * nout));
     // padded row stride: TvP = S (mod 32) spreads a footprint's rows over the banks
     gp.TvP = g.Tv + ((g.supp - g.Tv) % 32 + 32) % 32;
-    const std::size_t per_plane = std::size_t(nout) * g.Tu * gp.TvP * 2 * sizeof(GridAcc);
-    if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
-    // at most 32 planes per item: a staged record keeps its item plane in 5 bits
-    gp.G_planes = std::max(1, std::min({P, int(grid_lds<T>() / per_plane), 32}));
+    if (g.mfma) {
+        // the matrix cores hold kMfmaPlanes planes of a tile in registers
+        gp.G_planes = std::max(1, std::min(P, kMfmaPlanes));
+        gp.lds = 0;
+    } else {
+        const std::size_t per_plane = std::size_t(nout) * g.Tu * gp.TvP * 2 * sizeof(GridAcc);
+        if (per_plane > grid_lds<T>()) throw DevError{"tile too large for LDS"};
+        // at most 32 planes per item: a staged record keeps its item plane in 5 bits
+        gp.G_planes = std::max(1, std::min({P, int(grid_lds<T>() / per_plane), 32}));
+        gp.lds = per_plane * std::min(P, gp.G_planes);
+    }
     gp.ngroups = (P + gp.G_planes - 1) / gp.G_planes;
     gp.ntg = g.ntu * g.ntv * gp.ngroups;
-    gp.lds = per_plane * std::min(P, gp.G_planes);
     const int ntg = gp.ntg;
     std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (ntg + 1)));
     gp.item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (ntg + 1)));
@@ -3207,6 +3454,10 @@ void grid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const DevIn& 
     const std::uint32_t rc = kGridChunk;
     if (gp.ub == 0) {
         // nothing to grid: every cell was cleared above
+    } else if (g.mfma) {
+        if constexpr (sizeof(T) == 4)
+            launch_grid_mfma(g.supp, int(gp.ub), form, st, g, recs, off, gp.tb.coeff, din, gp.G_planes, gp.ngroups,
+                             rc, gp.item_off, gp.tb.order, gp.grid);
     } else if (nout == 2)
         launch_grid<T, 2, false>(g.supp, int(gp.ub), gp.lds, st, g, recs, off, gp.tb.coeff, din, gp.G_planes,
                                  gp.ngroups, gp.TvP, rc, gp.item_off, gp.tb.order, gp.grid);
@@ -3486,11 +3737,12 @@ struct BatchPool {
     }
 };
 
+std::mutex g_pool_mu;
+std::map<int, std::unique_ptr<BatchPool>> g_pool_all;
+
 BatchPool& batch_pool(int device) {
-    static std::mutex mu;
-    static std::map<int, std::unique_ptr<BatchPool>> all;
-    std::lock_guard<std::mutex> lk(mu);
-    auto& p = all[device];
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto& p = g_pool_all[device];
     if (!p) p.reset(new BatchPool());
     return *p;
 }
@@ -3573,9 +3825,9 @@ int device_batch_vis2dirty(BatchImage* imgs, int n, std::size_t budget, hipStrea
             while (end < order.size() && (end == at || used + plans[order[end]].bytes <= budget))
                 used += plans[order[end++]].bytes;
             std::vector<GridPrep<T>> preps(end - at);
-            std::vector<GridJob<T>> jobs;
+            std::vector<GridJob<T>> jobs, mjobs;  // LDS gridding, matrix-core gridding
             std::size_t lds = 0;
-            long long total = 0;
+            long long total = 0, mtotal = 0;
             for (std::size_t k = at; k < end; ++k) {
                 const int i = order[k];
                 Plan& pl = plans[i];
@@ -3587,23 +3839,36 @@ int device_batch_vis2dirty(BatchImage* imgs, int n, std::size_t budget, hipStrea
                 GridPrep<T>& gp = preps[k - at];
                 grid_prepare<T>(ws, pl.G, pl.g, off, 1, st, keep, timer, nullptr, gp);
                 if (gp.ub == 0) continue;
-                if (total + gp.ub >= (1LL << 31)) throw DevError{"too many gridding items in one batch"};
-                jobs.push_back(GridJob<T>{pl.g, recs, off, gp.tb.coeff, pl.din, gp.G_planes, gp.ngroups, gp.TvP,
-                                          kGridChunk, gp.item_off, gp.tb.order, gp.grid, std::uint32_t(total)});
-                total += gp.ub;
-                lds = std::max(lds, gp.lds);
+                long long& tot = pl.g.mfma ? mtotal : total;
+                if (tot + gp.ub >= (1LL << 31)) throw DevError{"too many gridding items in one batch"};
+                (pl.g.mfma ? mjobs : jobs)
+                    .push_back(GridJob<T>{pl.g, recs, off, gp.tb.coeff, pl.din, gp.G_planes, gp.ngroups, gp.TvP,
+                                          kGridChunk, gp.item_off, gp.tb.order, gp.grid, std::uint32_t(tot)});
+                tot += gp.ub;
+                if (!pl.g.mfma) lds = std::max(lds, gp.lds);
             }
             timer.mark("sort");
-            if (!jobs.empty()) {
-                keep.b.emplace_back(reinterpret_cast<const char*>(jobs.data()),
-                                    reinterpret_cast<const char*>(jobs.data() + jobs.size()));
-                const std::size_t jb = sizeof(GridJob<T>) * jobs.size();
-                void* dj = pool.control.get(kSlotItems, jb);
+            // the job tables to the device: one launch per gridding kernel
+            auto upload = [&](const std::vector<GridJob<T>>& js, int slot) {
+                keep.b.emplace_back(reinterpret_cast<const char*>(js.data()),
+                                    reinterpret_cast<const char*>(js.data() + js.size()));
+                const std::size_t jb = sizeof(GridJob<T>) * js.size();
+                void* dj = pool.control.get(slot, jb);
                 HIPCHK(hipMemcpyAsync(dj, keep.b.back().data(), jb, hipMemcpyHostToDevice, st));
+                return static_cast<const GridJob<T>*>(dj);
+            };
+            if (!jobs.empty()) {
+                const GridJob<T>* dj = upload(jobs, kSlotItems);
                 lds_limit((const void*)k_grid_jobs<T, 1, false>, lds);
-                k_grid_jobs<T, 1, false><<<int(total), grid_threads<T>(), lds, st>>>(
-                    static_cast<const GridJob<T>*>(dj), int(jobs.size()));
+                k_grid_jobs<T, 1, false><<<int(total), grid_threads<T>(), lds, st>>>(dj, int(jobs.size()));
                 HIPCHK(hipGetLastError());
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (!mjobs.empty()) {
+                    const GridJob<T>* dj = upload(mjobs, kSlotItemOff);
+                    k_grid_mfma_jobs<false><<<int(mtotal), 64, 0, st>>>(dj, int(mjobs.size()));
+                    HIPCHK(hipGetLastError());
+                }
             }
             timer.mark("grid");
             for (std::size_t k = at; k < end; ++k) {
@@ -4645,6 +4910,35 @@ int prepared_power(PreparedBase* base, std::uint64_t seed, double tolerance, std
 RANG_PREPARED_INST(float)
 RANG_PREPARED_INST(double)
 #undef RANG_PREPARED_INST
+
+// Frees the device buffers every call keeps between calls (the per-device
+// workspaces and the sub-band batch pools; not the fixed scratch block,
+// nor the buffers of live prepared plans) after the device's work is done.
+void release_workspaces() {
+    std::vector<Workspace*> all;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        for (auto& kv : g_ws_all)
+            if (kv.second) all.push_back(kv.second.get());
+    }
+    (void)hipDeviceSynchronize();
+    for (Workspace* w : all) {
+        std::lock_guard<std::mutex> lk(w->mu);
+        free_workspace(*w);
+    }
+    std::vector<BatchPool*> pools;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (auto& kv : g_pool_all)
+            if (kv.second) pools.push_back(kv.second.get());
+    }
+    for (BatchPool* p : pools) {
+        std::lock_guard<std::mutex> lk(p->mu);
+        free_workspace(p->control);
+        for (auto& w : p->images) free_workspace(*w);
+        p->images.clear();
+    }
+}
 
 template int device_batch_vis2dirty<float>(BatchImage*, int, std::size_t, hipStream_t, std::string&);
 template int device_batch_vis2dirty<double>(BatchImage*, int, std::size_t, hipStream_t, std::string&);

@@ -154,6 +154,7 @@ class GridOutputs(ctypes.Structure):
 GRID_N_STATS = 8
 GRID_N_TIMINGS = 14
 SIGNATURES = {
+    "rang_amd_release_workspaces": ([], _i32),
     "rang_amd_wgridder_batch_vis2dirty_f32": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),
     "rang_amd_wgridder_batch_vis2dirty_f64": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),
     "rang_amd_wgridder_tune_f32": (

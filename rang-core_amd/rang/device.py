@@ -98,3 +98,9 @@ def stage_times():
     lib = _core.library()
     return [(lib.rang_amd_stage_name(i).decode(), lib.rang_amd_stage_ms(i))
             for i in range(lib.rang_amd_stage_count())]
+
+
+def release():
+    """Frees the device memory the library keeps between calls (its
+    workspaces and sub-band batch pools; not live prepared plans)."""
+    _check(_core.library().rang_amd_release_workspaces())

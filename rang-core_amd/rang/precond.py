@@ -662,7 +662,8 @@ class FilteredTrainer:
     rms(M r^), y the network's output on (r^, PSF, M r^ / c); G is `gram`,
     an operator with apply(x, out) (the exact normal operator of the plan, or
     the normal operator without visibilities, rang.gram.GramOperator, which
-    costs no gridding). Counts G's applications."""
+    costs no gridding). Counts G's applications (gram_applies) and the exact
+    operator's (exact_applies, those of collect)."""
 
     def __init__(self, measurement, net, filt, gram, *, seed=7, components=(64, 24), widths=(0.5, 24.0)):
         self.m, self.net, self.filt, self.G = measurement, net, filt, gram
@@ -691,6 +692,7 @@ class FilteredTrainer:
         _ok(pack(_p(allpsf), None, _p(inp), CIN, 1, B, self.hw, _stream()))
         self.batches = 0
         self.gram_applies = 0
+        self.exact_applies = 0
         self.pool = []
 
     def _g(self, src, dst):
@@ -717,7 +719,7 @@ class FilteredTrainer:
         for j in range(rhs_count):
             self._sky(1)
             r = plan.gram(self.x[0].to(self.real).contiguous()).clone()
-            self.gram_applies += 1
+            self.exact_applies += 1
             x = torch.zeros_like(r)
             z = self.filt.apply(r)
             p = z.clone()
@@ -727,7 +729,7 @@ class FilteredTrainer:
                     rr = dot(r, r)
                     self.pool.append((r * (1.0 / math.sqrt(rr / hw))).contiguous())
                 q = plan.gram(p)
-                self.gram_applies += 1
+                self.exact_applies += 1
                 pq = dot(p, q)
                 if not pq > 0:
                     break

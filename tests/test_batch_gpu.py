@@ -116,3 +116,13 @@ def test_a_refused_image_is_named():
     images[1].opts[0] = 47
     with pytest.raises(ValueError, match="image 1"):
         batch.vis2dirty(images, single=True)
+
+
+def test_released_workspaces_are_allocated_again():
+    made = [make(s, True) for s in SHAPES[:3]]
+    images = [m[0] for m in made]
+    first = [d.cpu().numpy() for d in batch.vis2dirty(images, single=True)]
+    device.release()
+    again = [d.cpu().numpy() for d in batch.vis2dirty(images, single=True)]
+    for a, b in zip(first, again):
+        assert rel_linf(a, b) < 1e-6
