@@ -443,7 +443,7 @@ def cpu_large(info, nvis_total):
     }
 
 
-def published(dev):
+def published(dev, cpu_baseline=True):
     """The reference's own published timings at one size, beside this GPU's
     2016 baselines x 738 times x 16 channels =
     23,804,928 visibilities (the reference's whole-array cases have
@@ -496,6 +496,33 @@ def published(dev):
             raise RuntimeError(_core.wgridder_last_error())
     t_host = times[-1]
     progress(f"exact gridder f64 dirty+PSF: device {t_dev:.3f} s, host arrays {t_host:.3f} s")
+    # the oracle on this host beside the reference's published 1- and
+    # 16-thread times: all threads on the whole case, one thread on the
+    # first quarter of the rows (its time scaled by 4: the gridding's share
+    # dominates; the transforms of 7 planes of 2376^2 take well under a
+    # second)
+    oracle_cpu = None
+    if cpu_baseline:
+        from oracle import oracle as O
+
+        def oracle_run(rows):
+            t0 = time.perf_counter()
+            O.vis2dirty(uvw_h[:rows], freq_h, vis_h[:rows], wgt_h[:rows], npix_x=1024, npix_y=1024,
+                        pixsize_x=pix, pixsize_y=pix, epsilon=1e-6, single=False, outputs=("dirty", "psf"))
+            return time.perf_counter() - t0
+
+        threads = _threads()
+        O.set_threads(threads)
+        t_all = oracle_run(nrow)
+        O.set_threads(1)
+        quarter = nrow // 4
+        t_one = oracle_run(quarter) * nrow / quarter
+        O.set_threads(threads)
+        oracle_cpu = {"threads": threads, "seconds_all_threads": round(t_all, 3),
+                      "seconds_one_thread": round(t_one, 2),
+                      "one_thread_sample": f"the first {quarter} of {nrow} rows, time scaled by {nrow / quarter:.2f}",
+                      "reference_seconds_16_threads": 1.93, "reference_seconds_1_thread": 11.35}
+        progress(f"oracle f64 dirty+PSF: {t_all:.2f} s on {threads} threads, {t_one:.1f} s on one (scaled)")
     data = np.ascontiguousarray(c["vis"].cpu().numpy().view(np.float32).reshape(nrow, nchan, 2))
     w32 = np.ascontiguousarray(c["wgt"].cpu().numpy().astype(np.float32))
     ws = {}
@@ -525,6 +552,7 @@ def published(dev):
             "seconds_device": round(t_dev, 3), "seconds_host_arrays": round(t_host, 3),
             "reference_seconds_16_threads": 1.93, "ducc0_seconds_16_threads": 1.81,
             "speedup_host_arrays": round(1.93 / t_host, 1),
+            "oracle_on_this_host": oracle_cpu,
         },
         "wstack_64_layers": {
             "seconds_f64": ws["f64"], "seconds_f32": ws["f32"],
@@ -743,7 +771,7 @@ def main():
                     help="skip the cases the reference published timings for")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
-    ap.add_argument("--profile", choices=("large", "small"),
+    ap.add_argument("--profile", choices=("large", "small", "unet"),
                     help="only run --warmup + --steps full vis2dirty + dirty2vis calls of one workload and print "
                          "their stage times (for profiler runs; no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -771,6 +799,42 @@ def main():
 
     dev = torch.device("cuda:0")
     from rang import device as D
+
+    if args.profile == "unet":
+        # the preconditioner's network at the Gram-solve size: forward and
+        # backward passes of 4 images of 2048^2 on random inputs
+        from rang import precond as P
+
+        net = P.UNet(2048, 2048, batch=4, levels=4, base=32, seed=1)
+        net.buf[0]["in"].copy_(torch.randn(net.buf[0]["in"].shape, device=dev).to(torch.bfloat16)
+                               .view(torch.int16))
+        net.g_y.normal_()
+
+        def one():
+            net.forward(4)
+            net.zero_grad()
+            net.backward(4)
+
+        for _ in range(args.warmup):
+            one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            net.forward(4)
+        torch.cuda.synchronize()
+        t_f = (time.perf_counter() - t0) / max(args.steps, 1)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            one()
+        torch.cuda.synchronize()
+        t_fb = (time.perf_counter() - t0) / max(args.steps, 1)
+        fl = net.forward_flops(4)
+        print(json.dumps({"profile": "unet", "images": 4, "side": 2048, "forward_ms": round(t_f * 1e3, 3),
+                          "forward_tflops": round(fl / t_f / 1e12, 2),
+                          "forward_backward_ms": round(t_fb * 1e3, 3),
+                          "forward_backward_tflops": round(3 * fl / t_fb / 1e12, 2),
+                          "bf16_dense_peak_tflops": 2500.0}))
+        return
 
     if args.profile:
         if args.profile == "large":
@@ -858,7 +922,7 @@ def main():
             print(json.dumps({"config_small": small_line}))
             return
     cpu = None if args.no_cpu_baseline else cpu_large(info, nvis_total)
-    pub = None if args.no_published else published(dev)
+    pub = None if args.no_published else published(dev, not args.no_cpu_baseline)
     sub = None
     if not args.no_batch:
         sub = run_batch(dev)

@@ -50,9 +50,10 @@ int32_t rang_core_grid_invert(const GridInputs *inputs, GridOutputs *outputs);
 int32_t rang_core_grid_kernel(size_t kernel_size, size_t oversampling, double *kernel);
 
 /* core/src/gpu/ffi.rs:104-137: precision 0 double, 1 single; timings 14
- * doubles (context, compile, prepare, bucket, upload, compute, grid, fft,
- * project, download, finish seconds; device bytes, batches, layers per
- * batch); 0 ok, 1 invalid arguments, 2 a failure inside, 3 the GPU path
+ * doubles in the order of Timings::as_array (core/src/gpu/invert.rs:57-74):
+ * context, compile, prepare, upload, compute, download, finish seconds;
+ * device bytes, batches, layers per batch; grid, fft, project, bucket
+ * seconds. 0 ok, 1 invalid arguments, 2 a failure inside, 3 the GPU path
  * failed. As the reference's GPU path, padded sides with a prime factor
  * above 11 are refused (3). */
 int32_t rang_core_grid_invert_gpu(const GridInputs *inputs, GridOutputs *outputs, int32_t precision,

@@ -118,8 +118,11 @@ def lib():
 
 
 def set_threads(n):
-    """OpenMP threads of the oracle (it reads OMP_NUM_THREADS at first use)."""
+    """OpenMP threads of the oracle's later calls (OMP_NUM_THREADS before its
+    first use, omp_set_num_threads after)."""
     os.environ["OMP_NUM_THREADS"] = str(int(n))
+    if _lib is not None:
+        _lib.or_set_threads(int(n))
 
 
 def _p(a):

@@ -7,6 +7,7 @@
  * kernel choice and the call geometry. Each function names the reference
  * lines it follows (core/src/wgridder.rs).
  */
+#include <omp.h>
 #include "wg_common.h"
 
 #include <math.h>
@@ -73,6 +74,12 @@ int or_available_kernels(double eps, int ndim, int single, double ofmin, double 
 }
 
 /* wgridder.rs:185-222 (good_size_cmplx) */
+/* The OpenMP threads of later calls (the baseline's 1- and all-thread
+ * timings in one process). */
+void or_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
 size_t or_good_size_complex(size_t n) {
     if (n <= 12) return n;
     size_t best = 2 * n;

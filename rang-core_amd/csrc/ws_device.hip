@@ -871,9 +871,21 @@ int invert(const GridInputs& i, const GridOutputs& o, double* timings, std::stri
     std::memcpy(o.stats, stats, sizeof(stats));
     if (timings) {
         const double total = since(t_all);
-        const double tm[14] = {0.0, 0.0, t_prepare, t_bucket, t_upload, total - t_upload - t_prepare - t_download - t_finish,
-                               t_grid, t_fft, t_project, t_download, t_finish, double(b.bytes), double(nbatches),
-                               double(batch)};
+        // the order of Timings::as_array (core/src/gpu/invert.rs:57-74)
+        const double tm[14] = {0.0,
+                               0.0,
+                               t_prepare,
+                               t_upload,
+                               total - t_upload - t_prepare - t_download - t_finish,
+                               t_download,
+                               t_finish,
+                               double(b.bytes),
+                               double(nbatches),
+                               double(batch),
+                               t_grid,
+                               t_fft,
+                               t_project,
+                               t_bucket};
         std::memcpy(timings, tm, sizeof(tm));
     }
     return 0;

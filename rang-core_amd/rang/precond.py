@@ -150,6 +150,22 @@ class UNet:
     def parameters(self):
         return sum(k.w.numel() for k in self.convs)
 
+    def forward_flops(self, B):
+        """Multiply-add flops (2 per product) of the convolutions of one
+        forward pass over B images, as the matrix cores compute them (the
+        padded input and output channels included)."""
+        L, c = self.levels, self.c
+        total = 0
+        for l in range(L):
+            P = B * (self.H >> l) * (self.W >> l)
+            ka, kb = self.enc[l]
+            total += 2 * P * 9 * (ka.cin * ka.cout + kb.cin * kb.cout)
+            if self.dec[l]:
+                da, db = self.dec[l]
+                total += 2 * P * 9 * (da.cin * da.cout + db.cin * db.cout)
+        total += 2 * B * self.H * self.W * self.out.cin * self.out.cout
+        return total
+
     # ------------------------------------------------------------- forward
     def forward(self, B):
         """The output (channel 0 of self.y) of the inputs packed in buf[0]['in']."""

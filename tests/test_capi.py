@@ -106,5 +106,5 @@ def test_grid_kernel_and_padding_without_a_gpu():
     assert np.array_equal(core_grid.kernel(7, 1023), O.ws_kernel(7, 1023))
     # core/src/grid.rs:954-970
     assert (core_grid.padded_size(512), core_grid.padded_size(100)) == (616, 120)
-    with pytest.raises(RuntimeError, match="code 1"):
+    with pytest.raises(ValueError, match="invalid kernel settings"):
         core_grid.kernel(0, 1023)

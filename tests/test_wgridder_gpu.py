@@ -181,6 +181,18 @@ def test_explicit_kernel_and_wide_support():
         assert rel_linf(got, ref) < TOL_ORACLE[single]
 
 
+def test_more_than_32_planes_in_lds_tiles_of_the_support():
+    # f64, an explicit support-16 row on a 16 x 16 grid: the LDS gridder's
+    # tile is the support itself (no padding), so more than 32 planes would
+    # fit one item; an item holds at most 32 (a staged record keeps its item
+    # plane in 5 bits), and every plane must land where the oracle puts it
+    uvw, freq, vis, wgt, nx, ny, dx = ref_case(nx=8, ny=8, field=0.3, wmax=1500.0, nrow=20, seed=14)
+    got, ref, info, rinfo = v2d(uvw, freq, vis, wgt, nx, ny, dx, 1e-5, False, kernel=(773, 16, 16))
+    assert info["supp"] == 16 and info["nplanes"] > 64
+    same_plan(info, rinfo)
+    assert rel_linf(got, ref) < TOL_ORACLE[False]
+
+
 def test_masks_and_empty_inputs():
     uvw, freq, vis, wgt, nx, ny, dx = ref_case(nrow=30, seed=12)
     mask = (np.random.default_rng(1).uniform(size=vis.shape) > 0.4).astype(np.uint8)

@@ -8,6 +8,8 @@ own FFT), 2e-5 in single precision (grid and transforms in float, sums in
 double). Counters that are sums of float32 weights in double: relative
 1e-12; the gridded count and the w range: exact."""
 
+import math
+
 import numpy as np
 import pytest
 from conftest import ROOT
@@ -151,3 +153,44 @@ def test_any_padded_side(width, height, padding, lds_max, monkeypatch):
     assert rel(r["dirty"], ref["dirty"]) < TOL["f64"]
     assert rel(r["psf"], ref["psf"]) < TOL["f64"]
     same_stats(r["stats"], ref["stats"])
+
+
+def _point(l_pix, m_pix, width=48, scale_arcsec=60.0, seed=20261029):
+    # a unit point source (l_pix, m_pix) pixels from the centre through 80
+    # random baselines at two frequencies (the reference's
+    # tests/test_core_grid.py:37-55 case)
+    from rang import core_grid
+
+    scale = scale_arcsec * core_grid.ARCSEC
+    l, m = l_pix * scale, m_pix * scale
+    n = math.sqrt(1.0 - l * l - m * m)
+    freq = np.array([1.0e9, 1.1e9])
+    rng = np.random.default_rng(seed)
+    radius, angle = rng.uniform(5.0, 120.0, 80), rng.uniform(0.0, 2 * math.pi, 80)
+    uvw = np.stack([radius * np.cos(angle), radius * np.sin(angle), rng.normal(0, 4, 80)], 1)
+    phase = 2 * math.pi * (uvw[:, :1] * l + uvw[:, 1:2] * m + uvw[:, 2:3] * (n - 1)) * freq / 299792458.0
+    data = np.exp(1j * phase).astype(np.complex64)
+    w = np.ones(data.shape, np.float32)
+    return (uvw, freq, data, w, w), dict(width=width, height=width, scale_arcsec=scale_arcsec, w_layers=5)
+
+
+@pytest.mark.parametrize("gridder", ["wstack", "wgrid"])
+def test_point_source_at_the_wsclean_pixel_for_both_gridders(gridder, monkeypatch):
+    from rang import core_grid
+
+    monkeypatch.delenv(core_grid.DEVICE_ENV, raising=False)
+    args, kw = _point(7, -4)
+    r = core_grid.invert(*args, **kw, gridder=gridder)
+    dirty, psf, w = r["dirty"], r["psf"], kw["width"]
+    y, x = np.unravel_index(np.argmax(dirty), dirty.shape)
+    assert (x, y) == (w // 2 - 7, w // 2 - 4)
+    assert abs(psf[w // 2, w // 2] - 1.0) <= 1e-12
+    assert abs(dirty[y, x] - 1.0) < 0.05
+    assert r["gridder"] == gridder and "gpu" not in r
+    assert r["stats"]["gridded_count"] == args[2].size
+    if gridder == "wstack":
+        # the default device is the reference's CPU path (run here on the GPU)
+        explicit = core_grid.invert(*args, **kw, device="cpu")
+        assert np.array_equal(explicit["dirty"], dirty)
+    else:
+        assert r["wgridder"]["epsilon"] == core_grid.WGRID_EPSILON
