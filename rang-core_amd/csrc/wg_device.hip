@@ -2915,7 +2915,7 @@ void fft_dispatch(int need, F&& f) {
         f(ic<512>{}, ic<8192>{});
     else if constexpr (sizeof(T) == 4) {
         if (need <= 16384)
-            f(ic<512>{}, ic<16384>{});
+            f(ic<1024>{}, ic<16384>{});
         else
             throw DevError{"transform too long for LDS"};
     } else {

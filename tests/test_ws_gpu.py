@@ -189,8 +189,9 @@ def test_point_source_at_the_wsclean_pixel_for_both_gridders(gridder, monkeypatc
     assert r["gridder"] == gridder and "gpu" not in r
     assert r["stats"]["gridded_count"] == args[2].size
     if gridder == "wstack":
-        # the default device is the reference's CPU path (run here on the GPU)
+        # the default device is the reference's CPU path (run here on the GPU,
+        # whose atomic sums may differ between calls in the last bits)
         explicit = core_grid.invert(*args, **kw, device="cpu")
-        assert np.array_equal(explicit["dirty"], dirty)
+        assert "gridder" in explicit and rel(explicit["dirty"], dirty) <= TOL["f64"]
     else:
         assert r["wgridder"]["epsilon"] == core_grid.WGRID_EPSILON
