@@ -195,3 +195,21 @@ def test_point_source_at_the_wsclean_pixel_for_both_gridders(gridder, monkeypatc
         assert "gridder" in explicit and rel(explicit["dirty"], dirty) <= TOL["f64"]
     else:
         assert r["wgridder"]["epsilon"] == core_grid.WGRID_EPSILON
+
+
+def test_wide_field_8192_pads_to_a_side_with_a_large_prime():
+    # 8192^2 at 1.3 arcsec pads (1.2) to 9832 = 8 x 1229: the reference's CPU
+    # path takes it through rustfft, here through Bluestein transforms in
+    # four steps; a unit point source 300 x 200 pixels off centre lands on
+    # its WSClean pixel with the PSF centre at one
+    from rang import core_grid
+
+    assert core_grid.padded_size(8192) == 9832
+    args, kw = _point(300, -200, width=8192, scale_arcsec=1.3)
+    kw["w_layers"] = 8
+    r = core_grid.invert(*args, **kw, device="cpu")
+    dirty, psf, w = r["dirty"], r["psf"], 8192
+    y, x = np.unravel_index(np.argmax(dirty), dirty.shape)
+    assert (x, y) == (w // 2 - 300, w // 2 - 200)
+    assert abs(psf[w // 2, w // 2] - 1.0) <= 1e-12
+    assert abs(dirty[y, x] - 1.0) < 0.05
