@@ -771,7 +771,7 @@ def main():
                     help="skip the cases the reference published timings for")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
-    ap.add_argument("--profile", choices=("large", "small", "unet"),
+    ap.add_argument("--profile", choices=("large", "small", "unet", "wstack"),
                     help="only run --warmup + --steps full vis2dirty + dirty2vis calls of one workload and print "
                          "their stage times (for profiler runs; no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -834,6 +834,36 @@ def main():
                           "forward_backward_ms": round(t_fb * 1e3, 3),
                           "forward_backward_tflops": round(3 * fl / t_fb / 1e12, 2),
                           "bf16_dense_peak_tflops": 2500.0}))
+        return
+
+    if args.profile == "wstack":
+        # the w-stacking gridder on the published-comparison case (23.8 M
+        # visibilities, 1024^2 at 1.5 arcsec, 64 layers, kernel 7 x 1023,
+        # padding 1.2, dirty + PSF) from host arrays, f64 then f32
+        import numpy as np
+
+        from rang import _core
+
+        c = synth.imaging_case_device(738, 16, 1.30e9, 1.45e9, 1024, 1.5, dev)
+        nrow, nchan = int(c["uvw"].shape[0]), int(c["freq"].shape[0])
+        uvw_h, freq_h = c["uvw"].cpu().numpy(), c["freq"].cpu().numpy()
+        data = np.ascontiguousarray(c["vis"].cpu().numpy().view(np.float32).reshape(nrow, nchan, 2))
+        w32 = np.ascontiguousarray(c["wgt"].cpu().numpy().astype(np.float32))
+        out = {}
+        for prec in ("f64", "f32"):
+            d, p = np.zeros(1024 * 1024), np.zeros(1024 * 1024)
+            stats, timings = np.zeros(_core.GRID_N_STATS), np.zeros(_core.GRID_N_TIMINGS)
+            for k in range(args.warmup + args.steps):
+                t0 = time.perf_counter()
+                st = _core.grid_invert_gpu(nrow, nchan, 1024, 1024, 7, 1023, 64, c["pixsize"], c["pixsize"], 1.2,
+                                           uvw_h, freq_h, data, w32, w32, prec, d, p, stats, timings)
+                if st != 0:
+                    raise RuntimeError(_core.grid_last_error())
+                progress(f"w-stacking {prec} call {k + 1}: {time.perf_counter() - t0:.3f} s")
+            out[prec] = dict(zip(("context_s", "compile_s", "prepare_s", "upload_s", "compute_s", "download_s",
+                                  "finish_s", "gpu_bytes", "batches", "layers_per_batch", "grid_s", "fft_s",
+                                  "project_s", "bucket_s"), (float(v) for v in timings)))
+        print(json.dumps({"profile": "wstack", "timings": out}))
         return
 
     if args.profile:
