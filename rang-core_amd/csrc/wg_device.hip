@@ -2319,11 +2319,11 @@ __global__ __launch_bounds__((degrid_threads<T, STAGE>())) void k_degrid(Geo g, 
         }
     }
     __syncthreads();
-    const std::uint32_t total = b - a;
-    const std::uint32_t part = (total + blockDim.x - 1) / blockDim.x;
-    const std::uint32_t j0 = a + threadIdx.x * part, j1 = min(b, j0 + part);
+    // neighbouring lanes take neighbouring records (a row's channels, whose
+    // footprints overlap): their staged reads spread over the banks and their
+    // record loads coalesce
     double mis = 0.0;
-    for (std::uint32_t r = j0; r < j1; ++r) {
+    for (std::uint32_t r = a + threadIdx.x; r < b; r += blockDim.x) {
         const Rec<T> rc_ = recs[r];
         const int ou = int(rc_.pos & 255u), ov = int((rc_.pos >> 8) & 255u), mp = int(rc_.pos >> 16);
         T ku[S], kv[S];
