@@ -348,13 +348,16 @@ __device__ __forceinline__ cx<T> screen(double wk, int i, int j, int n, double d
 // Row i of conj(e_k) m, zero-padded to 2n, transformed forward; tmp holds n
 // rows of 2n.
 template <class T, int NT, int MT>
-__global__ __launch_bounds__(NT) void k_ap_rows_fwd(const T* __restrict__ m, int n, int use_w, double wk, double dx,
-                                                    double dy, cx<T>* __restrict__ tmp, FftDesc d,
+__global__ __launch_bounds__(NT) void k_ap_rows_fwd(const T* __restrict__ m, int n, int use_w, double w0, double dw,
+                                                    double dx, double dy, cx<T>* __restrict__ tmp_all, FftDesc d,
                                                     const cx<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(lds_raw);
     const int i = blockIdx.x;
     const int big = 2 * n;
+    // node blockIdx.y: its w and its own n x 2n slice of the half-transformed array
+    const double wk = w0 + double(blockIdx.y) * dw;
+    cx<T>* tmp = tmp_all + (long long)blockIdx.y * n * big;
     for (int j = threadIdx.x; j < big; j += NT) {
         cx<T> z{T(0), T(0)};
         if (j < n) {
@@ -377,11 +380,13 @@ __global__ __launch_bounds__(NT) void k_ap_rows_fwd(const T* __restrict__ m, int
 // Columns j0 .. j0 + C of tmp (n rows held, rows n .. 2n - 1 zero), forward,
 // times the spectrum, inverse, the first n rows written back.
 template <class T, int NT, int MT>
-__global__ __launch_bounds__(NT) void k_ap_cols(cx<T>* __restrict__ tmp, int n, const T* __restrict__ spec, int C,
-                                                FftDesc d, const cx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void k_ap_cols(cx<T>* __restrict__ tmp_all, int n, const T* const* __restrict__ specs,
+                                                int C, FftDesc d, const cx<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(lds_raw);
     const int big = 2 * n;
+    cx<T>* tmp = tmp_all + (long long)blockIdx.y * n * big;
+    const T* spec = specs[blockIdx.y];
     const int j0 = blockIdx.x * C;
     const int nc = min(C, big - j0);
     for (int e = threadIdx.x; e < nc * big; e += NT) {
@@ -403,28 +408,45 @@ __global__ __launch_bounds__(NT) void k_ap_cols(cx<T>* __restrict__ tmp, int n, 
     }
 }
 
-// Row i of tmp transformed back; out (+)= Re(y e_k) over its first n values.
+// Row i of every node's slice transformed back; out = sum over nodes of
+// Re(y_k e_k) over the first n values, summed in registers and written once.
 template <class T, int NT, int MT>
-__global__ __launch_bounds__(NT) void k_ap_rows_inv(const cx<T>* __restrict__ tmp, int n, int use_w, double wk, double dx,
-                                                    double dy, T* __restrict__ out, int first, FftDesc d,
-                                                    const cx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void k_ap_rows_inv(const cx<T>* __restrict__ tmp_all, int n, int nodes, int use_w,
+                                                    double w0, double dw, double dx, double dy, T* __restrict__ out,
+                                                    FftDesc d, const cx<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(lds_raw);
+    constexpr int PER = (MT / 2 + NT - 1) / NT;  // image values per thread (n <= MT / 2)
     const int i = blockIdx.x;
     const int big = 2 * n;
-    const cx<T>* row = tmp + (long long)i * big;
-    for (int j = threadIdx.x; j < big; j += NT) buf[j] = row[j];
-    __syncthreads();
-    fft_lds<T, NT, MT>(buf, d, 1, tw, +1);
-    for (int j = threadIdx.x; j < n; j += NT) {
-        const cx<T> y = buf[j];
-        T val = y.re;
-        if (use_w) {
-            const cx<T> e = screen<T>(wk, i, j, n, dx, dy);
-            val = y.re * e.re - y.im * e.im;
+    T acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = T(0);
+    for (int k = 0; k < nodes; ++k) {
+        const double wk = w0 + double(k) * dw;
+        const cx<T>* row = tmp_all + ((long long)k * n + i) * big;
+        for (int j = threadIdx.x; j < big; j += NT) buf[j] = row[j];
+        __syncthreads();
+        fft_lds<T, NT, MT>(buf, d, 1, tw, +1);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = int(threadIdx.x) + q * NT;
+            if (j < n) {
+                const cx<T> y = buf[j];
+                T val = y.re;
+                if (use_w) {
+                    const cx<T> e = screen<T>(wk, i, j, n, dx, dy);
+                    val = y.re * e.re - y.im * e.im;
+                }
+                acc[q] += val;
+            }
         }
-        T& o = out[(long long)i * n + j];
-        o = first ? val : o + val;
+        __syncthreads();  // the buffer is refilled by the next node
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = int(threadIdx.x) + q * NT;
+        if (j < n) out[(long long)i * n + j] = acc[q];
     }
 }
 
@@ -482,7 +504,9 @@ struct GramImpl final : GramBase {
     std::vector<T*> spectra;
     FftDesc d;
     cx<T>* tw = nullptr;
-    cx<T>* tmp = nullptr;
+    cx<T>* tmp = nullptr;      // n x 2n for the spectra's set-up
+    cx<T>* tmp_all = nullptr;  // one n x 2n slice per node for applications
+    T** d_spec = nullptr;      // the spectra's device addresses
     double st[kGramStats] = {};
     std::mutex mu;
 
@@ -492,6 +516,8 @@ struct GramImpl final : GramBase {
         for (T* s : spectra) (void)hipFree(s);
         if (tw) (void)hipFree(tw);
         if (tmp) (void)hipFree(tmp);
+        if (tmp_all) (void)hipFree(tmp_all);
+        if (d_spec) (void)hipFree(d_spec);
         if (cur != device) (void)hipSetDevice(cur);
     }
     bool single() const override { return sizeof(T) == 4; }
@@ -529,34 +555,40 @@ struct GramImpl final : GramBase {
         GCHK(hipGetLastError());
     }
 
+    // Every node in one launch per stage: the forward rows of all nodes, their
+    // columns (forward, times the node's spectrum, inverse), and the inverse
+    // rows summed over the nodes into `out`.
     void apply(const T* x, T* out, hipStream_t s) {
         const int big = 2 * n;
         const int C = cols_per_block();
-        for (std::size_t k = 0; k < spectra.size(); ++k) {
-            const double wk = w0 + double(k) * dw;
-            const int uw = use_w ? 1 : 0;
-            dispatch<T>(big, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds = sizeof(cx<T>) * big;
-                lds_attr(k_ap_rows_fwd<T, NT, MT>, lds);
-                k_ap_rows_fwd<T, NT, MT><<<n, NT, lds, s>>>(x, n, uw, wk, dx, dy, tmp, d, tw);
-            });
-            GCHK(hipGetLastError());
-            dispatch<T>(big * C, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds = sizeof(cx<T>) * big * C;
-                lds_attr(k_ap_cols<T, NT, MT>, lds);
-                k_ap_cols<T, NT, MT><<<(big + C - 1) / C, NT, lds, s>>>(tmp, n, spectra[k], C, d, tw);
-            });
-            GCHK(hipGetLastError());
-            dispatch<T>(big, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds = sizeof(cx<T>) * big;
-                lds_attr(k_ap_rows_inv<T, NT, MT>, lds);
-                k_ap_rows_inv<T, NT, MT><<<n, NT, lds, s>>>(tmp, n, uw, wk, dx, dy, out, k == 0 ? 1 : 0, d, tw);
-            });
-            GCHK(hipGetLastError());
+        const int K = int(spectra.size());
+        if (!tmp_all) {
+            GCHK(hipMalloc(&tmp_all, sizeof(cx<T>) * std::size_t(K) * n * big));
+            GCHK(hipMalloc(&d_spec, sizeof(T*) * std::size_t(K)));
+            GCHK(hipMemcpy(d_spec, spectra.data(), sizeof(T*) * std::size_t(K), hipMemcpyHostToDevice));
         }
+        const int uw = use_w ? 1 : 0;
+        dispatch<T>(big, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * big;
+            lds_attr(k_ap_rows_fwd<T, NT, MT>, lds);
+            k_ap_rows_fwd<T, NT, MT><<<dim3(n, K), NT, lds, s>>>(x, n, uw, w0, dw, dx, dy, tmp_all, d, tw);
+        });
+        GCHK(hipGetLastError());
+        dispatch<T>(big * C, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * big * C;
+            lds_attr(k_ap_cols<T, NT, MT>, lds);
+            k_ap_cols<T, NT, MT><<<dim3((big + C - 1) / C, K), NT, lds, s>>>(tmp_all, n, d_spec, C, d, tw);
+        });
+        GCHK(hipGetLastError());
+        dispatch<T>(big, [&](auto nt, auto mt) {
+            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+            const std::size_t lds = sizeof(cx<T>) * big;
+            lds_attr(k_ap_rows_inv<T, NT, MT>, lds);
+            k_ap_rows_inv<T, NT, MT><<<n, NT, lds, s>>>(tmp_all, n, K, uw, w0, dw, dx, dy, out, d, tw);
+        });
+        GCHK(hipGetLastError());
     }
 };
 
