@@ -1745,6 +1745,14 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
     const cx<T>* src = grid + row * nv;
     staged_fill<T, NT, MAXTOT>(buf, src, nv, [](int e) { return std::pair<long long, int>(e, e); });
     __syncthreads();
+    // a row no visibility reached (beyond the coverage on this plane)
+    // transforms to zero: write the zeros, skip the passes
+    int nz = 0;
+    for (int e = threadIdx.x; e < nv; e += NT) nz |= (buf[e].re != T(0) || buf[e].im != T(0)) ? 1 : 0;
+    if (!__syncthreads_or(nz)) {
+        for (int j = threadIdx.x; j < ny; j += NT) tmp[row * ny + j] = cx<T>{T(0), T(0)};
+        return;
+    }
     fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, +1);
     const int sh = nv - ny / 2;
     for (int j = threadIdx.x; j < ny; j += NT) {

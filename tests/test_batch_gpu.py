@@ -126,3 +126,18 @@ def test_released_workspaces_are_allocated_again():
     again = [d.cpu().numpy() for d in batch.vis2dirty(images, single=True)]
     for a, b in zip(first, again):
         assert rel_linf(a, b) < 1e-6
+
+
+def test_both_gridding_kernels_in_one_batch():
+    # a tile side asked for sends an image to the LDS gridder; the others go
+    # to the matrix cores: one launch each, every image still the oracle's
+    made = [make(s, True) for s in SHAPES[:5]]
+    images = [m[0] for m in made]
+    images[1].tile = 24
+    images[3].tile = 16
+    out = batch.vis2dirty(images, single=True)
+    torch.cuda.synchronize()
+    for (im, h), d in zip(made, out):
+        ref, rinfo = oracle_at(h, im.info, True)
+        assert list(im.info[:6]) == list(rinfo[:6])
+        assert rel_linf(d.cpu().numpy(), ref) < TOL[True]
