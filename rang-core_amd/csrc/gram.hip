@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(NT) void k_ap_cols(cx<T>* __restrict__ tmp_all, int
 template <class T, int NT, int MT>
 __global__ __launch_bounds__(NT) void k_ap_rows_inv(const cx<T>* __restrict__ tmp_all, int n, int nodes, int use_w,
                                                     double w0, double dw, double dx, double dy, T* __restrict__ out,
-                                                    FftDesc d, const cx<T>* __restrict__ tw) {
+                                                    int first, FftDesc d, const cx<T>* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(lds_raw);
     constexpr int PER = (MT / 2 + NT - 1) / NT;  // image values per thread (n <= MT / 2)
@@ -446,7 +447,10 @@ __global__ __launch_bounds__(NT) void k_ap_rows_inv(const cx<T>* __restrict__ tm
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int j = int(threadIdx.x) + q * NT;
-        if (j < n) out[(long long)i * n + j] = acc[q];
+        if (j < n) {
+            T& o = out[(long long)i * n + j];
+            o = first ? acc[q] : o + acc[q];
+        }
     }
 }
 
@@ -555,40 +559,64 @@ struct GramImpl final : GramBase {
         GCHK(hipGetLastError());
     }
 
-    // Every node in one launch per stage: the forward rows of all nodes, their
-    // columns (forward, times the node's spectrum, inverse), and the inverse
-    // rows summed over the nodes into `out`.
+    // The nodes in passes of nodes_per_pass(), one launch per stage and pass:
+    // the forward rows of the pass's nodes, their columns (forward, times the
+    // node's spectrum, inverse), and the inverse rows summed over the nodes
+    // into `out` (written by the first pass, added to by the others).
+    // Nodes per pass: their half-transformed slices within 2 GiB
+    // (RANG_GRAM_PASS_BYTES lowers the budget, a test knob; it is read at an
+    // operator's first application, which sizes the slices).
+    int pass_nodes = 0;
+    int nodes_per_pass() const {
+        std::size_t budget = std::size_t(2) << 30;
+        if (const char* e = std::getenv("RANG_GRAM_PASS_BYTES")) {
+            const long long v = std::atoll(e);
+            if (v > 0 && std::size_t(v) < budget) budget = std::size_t(v);
+        }
+        const std::size_t slice = sizeof(cx<T>) * std::size_t(n) * 2 * n;
+        const std::size_t k = std::max<std::size_t>(1, budget / slice);
+        return int(std::min<std::size_t>(k, spectra.size()));
+    }
+
     void apply(const T* x, T* out, hipStream_t s) {
         const int big = 2 * n;
         const int C = cols_per_block();
         const int K = int(spectra.size());
         if (!tmp_all) {
-            GCHK(hipMalloc(&tmp_all, sizeof(cx<T>) * std::size_t(K) * n * big));
+            pass_nodes = nodes_per_pass();
+            GCHK(hipMalloc(&tmp_all, sizeof(cx<T>) * std::size_t(pass_nodes) * n * big));
             GCHK(hipMalloc(&d_spec, sizeof(T*) * std::size_t(K)));
             GCHK(hipMemcpy(d_spec, spectra.data(), sizeof(T*) * std::size_t(K), hipMemcpyHostToDevice));
         }
+        const int KP = pass_nodes;
         const int uw = use_w ? 1 : 0;
-        dispatch<T>(big, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds = sizeof(cx<T>) * big;
-            lds_attr(k_ap_rows_fwd<T, NT, MT>, lds);
-            k_ap_rows_fwd<T, NT, MT><<<dim3(n, K), NT, lds, s>>>(x, n, uw, w0, dw, dx, dy, tmp_all, d, tw);
-        });
-        GCHK(hipGetLastError());
-        dispatch<T>(big * C, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds = sizeof(cx<T>) * big * C;
-            lds_attr(k_ap_cols<T, NT, MT>, lds);
-            k_ap_cols<T, NT, MT><<<dim3((big + C - 1) / C, K), NT, lds, s>>>(tmp_all, n, d_spec, C, d, tw);
-        });
-        GCHK(hipGetLastError());
-        dispatch<T>(big, [&](auto nt, auto mt) {
-            constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-            const std::size_t lds = sizeof(cx<T>) * big;
-            lds_attr(k_ap_rows_inv<T, NT, MT>, lds);
-            k_ap_rows_inv<T, NT, MT><<<n, NT, lds, s>>>(tmp_all, n, K, uw, w0, dw, dx, dy, out, d, tw);
-        });
-        GCHK(hipGetLastError());
+        for (int k0 = 0; k0 < K; k0 += KP) {
+            const int kn = std::min(KP, K - k0);
+            const double wk0 = w0 + double(k0) * dw;
+            dispatch<T>(big, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds = sizeof(cx<T>) * big;
+                lds_attr(k_ap_rows_fwd<T, NT, MT>, lds);
+                k_ap_rows_fwd<T, NT, MT><<<dim3(n, kn), NT, lds, s>>>(x, n, uw, wk0, dw, dx, dy, tmp_all, d, tw);
+            });
+            GCHK(hipGetLastError());
+            dispatch<T>(big * C, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds = sizeof(cx<T>) * big * C;
+                lds_attr(k_ap_cols<T, NT, MT>, lds);
+                k_ap_cols<T, NT, MT><<<dim3((big + C - 1) / C, kn), NT, lds, s>>>(tmp_all, n, d_spec + k0, C, d,
+                                                                                  tw);
+            });
+            GCHK(hipGetLastError());
+            dispatch<T>(big, [&](auto nt, auto mt) {
+                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                const std::size_t lds = sizeof(cx<T>) * big;
+                lds_attr(k_ap_rows_inv<T, NT, MT>, lds);
+                k_ap_rows_inv<T, NT, MT><<<n, NT, lds, s>>>(tmp_all, n, kn, uw, wk0, dw, dx, dy, out, k0 == 0 ? 1 : 0,
+                                                            d, tw);
+            });
+            GCHK(hipGetLastError());
+        }
     }
 };
 

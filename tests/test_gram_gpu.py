@@ -131,3 +131,22 @@ def test_inverse_filter_preconditions_cg(single):
         assert true_rel < 2 * tol
         best = n_p if best is None else min(best, n_p)
     assert rel_cg <= tol and best < n_cg
+
+
+def test_nodes_in_several_passes_give_the_same_operator(monkeypatch):
+    # the applications process the w-nodes in passes whose scratch fits a
+    # budget; a budget of one node per pass must give the one-pass result
+    from rang import gram
+
+    n = 32
+    c = gram_case(n, 800, 2000.0, False)
+    uvw, freq, wgt = dev(c)
+    x = torch.from_numpy(model_of(n)).cuda()
+    ex = gram.WExpansion(10, 1.0, 1e-13)
+    one = gram.GramOperator.new(opts(n, c["cell"], 1e-10), uvw, freq, wgt, ex, 1e-10, single=False)
+    want = one.apply(x).cpu().numpy()
+    assert one.stats()["planes"] > 2
+    monkeypatch.setenv("RANG_GRAM_PASS_BYTES", "1")
+    many = gram.GramOperator.new(opts(n, c["cell"], 1e-10), uvw, freq, wgt, ex, 1e-10, single=False)
+    got = many.apply(x).cpu().numpy()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
