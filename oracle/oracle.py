@@ -436,3 +436,48 @@ def imaging_weights(uvw, wavelength, weights, selected, *, mode="briggs", robust
     if st:
         raise ValueError("invalid imaging-weight arguments")
     return out, grid, sums
+
+
+def _blosc_lib():
+    L = lib()
+    if not getattr(L, "_blosc_bound", False):
+        vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        L.or_blosc_header.argtypes = [vp, sz, vp]
+        L.or_blosc_header.restype = i32
+        L.or_blosc_decode.argtypes = [vp, sz, vp, sz]
+        L.or_blosc_decode.restype = i32
+        L.or_lz4_block.argtypes = [vp, sz, vp, sz]
+        L.or_lz4_block.restype = i32
+        L._blosc_bound = True
+    return L
+
+
+def blosc_header(chunk):
+    """(nbytes, blocksize, nblocks, typesize, flags, cbytes) of a blosc-1
+    chunk, or None when it is refused (oracle/blosc.c)."""
+    b = np.frombuffer(bytes(chunk), np.uint8)
+    f = np.zeros(6, np.uint64)
+    if _blosc_lib().or_blosc_header(_p(b), b.size, _p(f)):
+        return None
+    return tuple(int(v) for v in f)
+
+
+def blosc_decode(chunk):
+    """The decoded bytes of a whole blosc-1 chunk, or None when refused."""
+    h = blosc_header(chunk)
+    if h is None:
+        return None
+    b = np.frombuffer(bytes(chunk), np.uint8)
+    out = np.zeros(max(h[0], 1), np.uint8)
+    if _blosc_lib().or_blosc_decode(_p(b), b.size, _p(out), h[0]):
+        return None
+    return out[: h[0]].tobytes()
+
+
+def lz4_block(src, n):
+    """One LZ4 block expanded to exactly `n` bytes, or None when corrupt."""
+    b = np.frombuffer(bytes(src), np.uint8)
+    out = np.zeros(max(n, 1), np.uint8)
+    if _blosc_lib().or_lz4_block(_p(b) if b.size else None, b.size, _p(out), n):
+        return None
+    return out[:n].tobytes()

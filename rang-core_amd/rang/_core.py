@@ -151,9 +151,43 @@ class GridOutputs(ctypes.Structure):
     _fields_ = [("dirty", ctypes.c_void_p), ("psf", ctypes.c_void_p), ("stats", ctypes.c_void_p)]
 
 
+class BloscChunk(ctypes.Structure):
+    """`RangBloscChunk` (include/rang_core_blosc.h)."""
+
+    _fields_ = [("src", ctypes.c_void_p), ("len", ctypes.c_size_t), ("dst", ctypes.c_void_p),
+                ("dst_len", ctypes.c_size_t)]
+
+
+class MsColumn(ctypes.Structure):
+    """`MsColumn` (core/src/ffi_ms.rs:24-33)."""
+
+    _fields_ = [("data", ctypes.c_void_p), ("nbytes", ctypes.c_size_t), ("nrows", ctypes.c_uint64),
+                ("ndim", ctypes.c_size_t), ("shape", ctypes.c_int64 * 8), ("dtype", ctypes.c_int32),
+                ("nstrings", ctypes.c_size_t)]
+
+
 GRID_N_STATS = 8
 GRID_N_TIMINGS = 14
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_colp = ctypes.POINTER(MsColumn)
 SIGNATURES = {
+    "rang_core_ms_last_error": ([_vp, _sz], _sz),
+    "rang_core_ms_open": ([ctypes.c_char_p, ctypes.POINTER(_vp)], _i32),
+    "rang_core_ms_close": ([_vp], None),
+    "rang_core_ms_describe": ([_vp, _vp, _sz], _sz),
+    "rang_core_ms_read": ([_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, _colp], _i32),
+    "rang_core_ms_free": ([_colp], None),
+    "rang_amd_ms_read_slice": ([_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, _i64p, _sz, _colp],
+                               _i32),
+    "rang_amd_ms_read_rows": ([_vp, ctypes.c_char_p, _u64p, _sz, _i64p, _sz, _colp], _i32),
+    "rang_amd_ms_read_dev": ([_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, _i64p, _sz, _vp, _sz,
+                              _vp, _colp], _i32),
+    "rang_core_blosc_size": ([_vp, _sz, ctypes.POINTER(_sz)], _i32),
+    "rang_core_blosc_header": ([_vp, _sz, ctypes.POINTER(_sz)], _i32),
+    "rang_core_blosc_decode": ([_vp, _sz, _vp, _sz], _i32),
+    "rang_amd_blosc_decode_batch": ([_vp, _sz, _vp], _i32),
+    "rang_amd_blosc_last_error": ([_vp, _sz], _sz),
     "rang_amd_release_workspaces": ([], _i32),
     "rang_amd_wgridder_batch_vis2dirty_f32": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),
     "rang_amd_wgridder_batch_vis2dirty_f64": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),

@@ -12,7 +12,8 @@ from rang import _core, core_wgridder
 
 HEADERS = [ROOT / "include" / "rang_core_wgridder.h", ROOT / "include" / "rang_core_grid.h",
            ROOT / "include" / "rang_core_precond.h", ROOT / "include" / "rang_core_gram.h",
-           ROOT / "include" / "rang_core_weights.h", ROOT / "include" / "rang_core_batch.h"]
+           ROOT / "include" / "rang_core_weights.h", ROOT / "include" / "rang_core_batch.h",
+           ROOT / "include" / "rang_core_ms.h", ROOT / "include" / "rang_core_blosc.h"]
 
 
 def declared():
@@ -22,7 +23,7 @@ def declared():
 
 def test_library_exports_every_declared_symbol():
     names = declared()
-    assert len(names) == 102
+    assert len(names) == 116
     lib = _core.library()
     for name in names:
         assert hasattr(lib, name), name
@@ -64,6 +65,42 @@ def test_batch_structs_match_a_c_compiler(tmp_path):
     B = _core.BatchImage
     assert got == [ctypes.sizeof(B), B.nrow.offset, B.vis_kind.offset, B.tile.offset, B.info.offset,
                    ctypes.sizeof(_core.TuneReport)]
+
+
+def test_ingest_structs_match_a_c_compiler(tmp_path):
+    # MsColumn (core/src/ffi_ms.rs:24-33) and RangBloscChunk as gcc lays them out
+    import ctypes
+    import subprocess
+
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "rang_core_ms.h"\n#include "rang_core_blosc.h"\n'
+        "int main(void) { printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(MsColumn), offsetof(MsColumn, shape),"
+        " offsetof(MsColumn, dtype), offsetof(MsColumn, nstrings), sizeof(RangBloscChunk)); return 0; }\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    M = _core.MsColumn
+    assert got == [ctypes.sizeof(M), M.shape.offset, M.dtype.offset, M.nstrings.offset,
+                   ctypes.sizeof(_core.BloscChunk)]
+
+
+def test_blosc_header_refusals_without_a_gpu():
+    # header parsing is host work: the reference's refusals (core/src/blosc.rs:53-79)
+    import ctypes
+    import struct
+
+    lib = _core.library()
+    n = ctypes.c_size_t()
+    good = struct.pack("<BBBBIII", 2, 1, 0x21, 8, 0, 256, 16)
+    assert lib.rang_core_blosc_size(good, len(good), ctypes.byref(n)) == 0 and n.value == 0
+    fields = (ctypes.c_size_t * 4)()
+    assert lib.rang_core_blosc_header(good, len(good), fields) == 0 and list(fields) == [0, 256, 0, 8]
+    for bad in (struct.pack("<BBBBIII", 2, 1, 0x25, 8, 0, 256, 16),   # bit shuffle
+                struct.pack("<BBBBIII", 2, 1, 0x41, 8, 0, 256, 16),   # not LZ4
+                struct.pack("<BBBBIII", 9, 1, 0x21, 8, 0, 256, 16),   # version
+                good[:12]):
+        assert lib.rang_core_blosc_size(bad, len(bad), ctypes.byref(n)) == 1
 
 
 def test_kernel_rows_through_the_abi_are_the_table():
