@@ -23,7 +23,8 @@ gridding launch per call) on tuned plans at epsilon 1e-5 and 1e-4
 seconds that plain CG, the inverse-filter PCG and the inner-outer solve (with
 and without the learned warm start) need to reach a relative residual of
 1e-4 on a 2048^2 image through 19.4 M visibilities (--no-gram skips it);
-under "published", the
+under "ingest", blosc-1 zarr chunks expanded on the GPU (the MSv4 ingest
+path; --no-ingest skips it); under "published", the
 exact gridder and the w-stacking gridder on a 23.8 M-visibility 1024^2 case
 beside the reference's published timings for that size (--no-published
 skips it).
@@ -290,11 +291,11 @@ def run_host_arrays(case, steps=1):
             "boundary": "rang_core_wgridder_vis2dirty_ex_f32 + dirty2vis_ex_f32 on host arrays (PCIe both ways)"}
 
 
-def profiled_traffic(workload, kernel):
+def profiled_traffic(workload, kernel, path=None):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE scaled by a plain
     copy's reading), or None."""
-    path = ROOT / "profiles" / f"wgridder_{workload}_traffic.md"
+    path = path or ROOT / "profiles" / f"wgridder_{workload}_traffic.md"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -757,6 +758,96 @@ def gram_solve(dev, tol=1e-4, budget=4000, train_steps=60, tau=0.1):
     }
 
 
+def ingest_chunk(blocks=16, blocksize=1 << 20, seed=3):
+    """A synthetic zarr chunk of a visibility variable: `blocks` blocks of
+    1 MiB of complex64 values (unit normal, 20 % zeros), byte-shuffled and
+    split into 8 LZ4 streams per block (c-blosc 1.21's layout at typesize 8
+    for 1 MiB blocks); one block is encoded (tests/blosc_cases.py's writer)
+    and stored `blocks` times. Returns (decoded bytes of one block, chunk)."""
+    import struct
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import blosc_cases as BC
+
+    data = BC.contents("vis", blocksize, seed)
+    one = BC.blosc_chunk(data, 8, blocksize, shuffled=True)
+    body = one[20:]
+    head = 16 + 4 * blocks
+    cbytes = head + blocks * len(body)
+    chunk = (struct.pack("<BBBBIII", 2, 1, one[2], 8, blocks * blocksize, blocksize, cbytes)
+             + b"".join(struct.pack("<I", head + k * len(body)) for k in range(blocks)) + body * blocks)
+    return data, chunk
+
+
+def run_ingest(dev, nchunks=64, steps=5, warmup=1, cpu_baseline=True):
+    """Blosc-1 chunks expanded on the GPU (the MSv4 ingest path): `nchunks`
+    chunks of 16 MiB decoded in one call, from device-resident compressed
+    bytes and from host bytes (PCIe-inclusive), beside the oracle's decoder
+    on one host thread."""
+    import numpy as np
+    import torch
+
+    from rang import msv4
+
+    block, chunk = ingest_chunk()
+    nbytes = 16 * len(block)
+    chunks = [chunk] * nchunks
+    outs = [torch.empty(nbytes, dtype=torch.uint8, device=dev) for _ in range(nchunks)]
+    res = torch.from_numpy(np.frombuffer(chunk, np.uint8).copy()).to(dev)
+    resident = [res] * nchunks
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    for _ in range(warmup):
+        msv4.blosc_decode_device(chunks, outs, resident)
+    t_res = timed(lambda: msv4.blosc_decode_device(chunks, outs, resident), steps)
+    want = np.frombuffer(block, np.uint8)
+    ok = all(np.array_equal(outs[i][k * len(block):(k + 1) * len(block)].cpu().numpy(), want)
+             for i in (0, nchunks - 1) for k in (0, 15))
+    t_host = timed(lambda: msv4.blosc_decode_device(chunks, outs), max(2, steps // 2))
+    total_out = nchunks * nbytes
+    total_in = nchunks * len(chunk)
+    alg = total_in + total_out
+    out = {
+        "workload": f"{nchunks} blosc-1 chunks of 16 MiB (16 blocks of 1 MiB complex64, byte shuffle, 8 LZ4 "
+                    "streams per block), decoded in one call",
+        "decoded_bytes": total_out,
+        "compressed_bytes": total_in,
+        "ratio": round(total_out / total_in, 4),
+        "resident": {"ms": round(t_res * 1e3, 3), "decoded_GBps": round(total_out / t_res / 1e9, 2)},
+        "host_bytes": {"ms": round(t_host * 1e3, 3), "decoded_GBps": round(total_out / t_host / 1e9, 2),
+                       "note": "compressed bytes uploaded from pageable host memory inside the call"},
+        "roofline": {"bound": "hbm", "achieved": round(alg / t_res / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / t_res / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("ingest", "k_lz4_streams",
+                                                 ROOT / "profiles" / "blosc_ingest_decode.md"),
+                     "note": "algorithmic bytes: compressed read once + decoded written once, over the "
+                             "whole call (host plan, table upload, LZ4 and unshuffle kernels); traffic: "
+                             "k_lz4_streams per launch"},
+        "parity": "equal" if ok else "MISMATCH",
+        "cpu_baseline": None,
+    }
+    if cpu_baseline:
+        from oracle import oracle as O
+
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            got = O.blosc_decode(chunk)
+        t_cpu = (time.perf_counter() - t0) / reps
+        assert got[: len(block)] == block
+        out["cpu_baseline"] = {"value": round(nbytes / t_cpu / 1e9, 3), "unit": "decoded GB/s", "cores": 1,
+                               "kind": "port", "sample": "one 16 MiB chunk, three times, the oracle's scalar "
+                               "decoder (oracle/blosc.c)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -771,7 +862,8 @@ def main():
                     help="skip the cases the reference published timings for")
     ap.add_argument("--only-small", action="store_true",
                     help="measure the small workload alone (no JSON contract line)")
-    ap.add_argument("--profile", choices=("large", "small", "unet", "wstack"),
+    ap.add_argument("--no-ingest", action="store_true", help="skip the blosc ingest leg")
+    ap.add_argument("--profile", choices=("large", "small", "unet", "wstack", "ingest"),
                     help="only run --warmup + --steps full vis2dirty + dirty2vis calls of one workload and print "
                          "their stage times (for profiler runs; no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -834,6 +926,11 @@ def main():
                           "forward_backward_ms": round(t_fb * 1e3, 3),
                           "forward_backward_tflops": round(3 * fl / t_fb / 1e12, 2),
                           "bf16_dense_peak_tflops": 2500.0}))
+        return
+
+    if args.profile == "ingest":
+        print(json.dumps({"profile": "ingest", **run_ingest(dev, steps=args.steps, warmup=args.warmup,
+                                                            cpu_baseline=False)}))
         return
 
     if args.profile == "wstack":
@@ -958,6 +1055,11 @@ def main():
         sub = run_batch(dev)
         D.release()
         torch.cuda.empty_cache()
+    ingest = None
+    if not args.no_ingest:
+        ingest = run_ingest(dev, cpu_baseline=not args.no_cpu_baseline)
+        progress(f"ingest: {ingest['resident']['decoded_GBps']} GB/s decoded")
+        torch.cuda.empty_cache()
     gram = None
     if not args.no_gram:
         gram = gram_solve(dev)
@@ -999,6 +1101,7 @@ def main():
         "cpu_baseline": cpu,
         "config_small": small_line,
         "sub_band_batch": sub,
+        "ingest": ingest,
         "gram_solve": gram,
         "published": pub,
     }

@@ -39,6 +39,11 @@ int32_t rang_core_blosc_header(const uint8_t *head, size_t len, size_t *fields);
 int32_t rang_core_blosc_decode(const uint8_t *src, size_t len, uint8_t *out, size_t out_len);
 
 int32_t rang_amd_blosc_decode_batch(const RangBloscChunk *chunks, size_t n, void *stream);
+/* As rang_amd_blosc_decode_batch with each chunk's bytes already in device
+ * memory at dev_src[i] (the host copy at chunks[i].src is read for the plan
+ * only: headers, block starts and stream lengths); nothing is uploaded. */
+int32_t rang_amd_blosc_decode_resident(const RangBloscChunk *chunks, const void *const *dev_src, size_t n,
+                                       void *stream);
 size_t rang_amd_blosc_last_error(uint8_t *buf, size_t len);
 
 #ifdef __cplusplus

@@ -7,11 +7,10 @@
 // every stream at once, one wave per stream: the sequences of a stream are
 // parsed by the whole wave in step (the stream's bytes staged in LDS a
 // window at a time), each literal run and match copied by the 64 lanes
-// together. A match (offset o, length m) is periodic with period o, so
-// lane i copies byte d - o + (i mod o): every byte it reads lies before the
-// match, written by earlier sequences, and waiting for the wave's stores
-// (with a work-group fence) makes those writes visible before the first
-// such read. Blocks stored shuffled are
+// together, its recent output kept in an LDS ring. A match (offset o,
+// length m) is periodic with period o, so lane i copies byte
+// d - o + (i mod o): every byte it reads lies before the match, written by
+// earlier sequences. Blocks stored shuffled are
 // expanded into scratch and then transposed (stream b holds byte b of every
 // element; core/src/blosc.rs:180-196).
 #include <hip/hip_runtime.h>
@@ -32,7 +31,6 @@ constexpr int kHead = 16;
 constexpr unsigned kShuffle = 0x01, kMemcpyed = 0x02, kBitShuffle = 0x04, kDontSplit = 0x10;
 constexpr int kMinBuffer = 128, kMaxSplits = 16;
 constexpr int kWaves = 4;         // streams per workgroup
-constexpr int kWindow = 2048;     // bytes of a stream staged in LDS at a time
 constexpr int kSlack = 24;        // bytes a sequence head may need past its token
 
 thread_local std::string t_blosc_error;
@@ -94,39 +92,81 @@ __device__ int g_blosc_status;  // first failure code of a launch (0 none)
 
 __device__ __forceinline__ void flag_error(int code) { atomicCAS(&g_blosc_status, 0, code); }
 
-// One wave expands one stream. The stream's bytes [wbase, wbase + kWindow)
-// sit in `win`; the parse (token, lengths, offset) reads there, the literal
-// copies read global memory.
+// One wave expands one stream. The stream's bytes [wbase, wend) sit in the
+// wave's LDS window (refilled with coalesced loads) and its last kRing
+// output bytes in the wave's LDS ring. A sequence's token, lengths and
+// offset are read from the window and made wave-uniform
+// (readfirstlane), so the parse runs on the scalar unit; its literals are
+// copied from the window, its match from the ring (a match of offset o
+// reads bytes [d - o, d), copied in segments of at most kRing - o bytes so
+// that no write lands on a byte still to be read), each byte stored to
+// global memory and to the ring. A match reaching further back than the
+// ring reads the wave's own earlier output in global memory: every store
+// older than the wave's last 63 vector memory instructions has completed
+// after `s_waitcnt vmcnt(63)`, and those instructions wrote at most
+// 63 x 64 < kRing bytes, so the bytes it reads are complete; the loads are
+// work-group-scope atomic loads, coherent with the wave's completed stores.
+constexpr int kRing = 4096;    // output bytes kept per wave (a power of two, > 63 x 64)
+constexpr int kWin = 1024;     // input bytes staged per wave
+constexpr int kWinPad = 0;
+constexpr int kLdsBytes = kWaves * (kRing + kWin + kWinPad);
+static_assert(kRing > 63 * 64, "the ring must cover the last 63 stores");
+
+__device__ __forceinline__ std::uint32_t uniform(std::uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Global-memory bytes: global (not flat) loads and stores, which count in
+// vmcnt only, so waits for the LDS reads of the parse do not wait for them.
+typedef __attribute__((address_space(1))) std::uint8_t gbyte;
+
 __global__ __launch_bounds__(64 * kWaves) void k_lz4_streams(const Stream* __restrict__ streams, int nstreams) {
-    __shared__ std::uint8_t lds[kWaves][kWindow];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    extern __shared__ __align__(16) std::uint8_t smem[];
+    // the wave's index and its stream are wave-uniform: the stream's
+    // description is read with scalar loads into scalar registers
+    const int lane = threadIdx.x & 63, w = int(uniform(threadIdx.x >> 6));
     const int sid = blockIdx.x * kWaves + w;
     if (sid >= nstreams) return;
+    std::uint8_t* ring = smem + w * (kRing + kWin + kWinPad);
+    std::uint8_t* win = ring + kRing;
     const Stream S = streams[sid];
-    const std::uint8_t* in = reinterpret_cast<const std::uint8_t*>(S.src);
-    std::uint8_t* out = reinterpret_cast<std::uint8_t*>(S.dst);
-    const std::uint32_t clen = S.clen, dlen = S.dlen;
+    const gbyte* in = reinterpret_cast<const gbyte*>(S.src);
+    gbyte* out = reinterpret_cast<gbyte*>(S.dst);
+    const std::uint32_t clen = uniform(S.clen), dlen = uniform(S.dlen);
     if (clen == dlen) {
-        // stored raw
-        for (std::uint32_t i = lane; i < dlen; i += 64) out[i] = in[i];
+        // stored raw: 16 loads in flight per lane (indices clamped into the
+        // stream, so every load is in bounds), then the stores
+        constexpr int U = 16;
+        for (std::uint32_t base = 0; base < dlen; base += 64 * U) {
+            std::uint8_t v[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) v[k] = in[min(base + lane + 64 * k, dlen - 1)];
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+                if (base + lane + 64 * k < dlen) out[base + lane + 64 * k] = v[k];
+        }
         return;
     }
-    std::uint8_t* win = lds[w];
+    constexpr std::uint32_t mask = kRing - 1;
     std::uint32_t wbase = 0, wend = 0;
+    // the window at `at` (< clen): every lane's loads issued before the
+    // first LDS write, indices clamped into the stream
     auto refill = [&](std::uint32_t at) {
         wbase = at;
-        wend = min(clen, at + std::uint32_t(kWindow));
-        for (std::uint32_t i = lane; i < wend - wbase; i += 64) win[i] = in[wbase + i];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        wend = min(clen, at + std::uint32_t(kWin));
+        const std::uint32_t n = wend - wbase;
+        std::uint8_t v[kWin / 64];
+#pragma unroll
+        for (int k = 0; k < kWin / 64; ++k) v[k] = in[wbase + min(std::uint32_t(lane + 64 * k), n - 1)];
+#pragma unroll
+        for (int k = 0; k < kWin / 64; ++k)
+            if (std::uint32_t(lane + 64 * k) < n) win[lane + 64 * k] = v[k];
     };
-    // byte `at` of the stream (at < clen), from the window when it holds it
+    // byte `at` of the stream (at < clen), from the window, wave-uniform
     auto byte = [&](std::uint32_t at) -> std::uint32_t {
         if (at >= wend || at < wbase) refill(at);
-        return win[at - wbase];
+        return uniform(win[at - wbase]);
     };
     refill(0);
-    std::uint32_t s = 0, d = 0, visible = 0;
+    std::uint32_t s = 0, d = 0;
     for (;;) {
         if (s >= clen) return flag_error(3);  // a stream ends with literals
         if (s + kSlack > wend && wend < clen) refill(s);
@@ -141,9 +181,30 @@ __global__ __launch_bounds__(64 * kWaves) void k_lz4_streams(const Stream* __res
             } while (b == 255);
         }
         if (lit > clen - s || lit > dlen - d) return flag_error(3);
-        for (std::uint32_t i = lane; i < lit; i += 64) out[d + i] = in[s + i];
-        s += lit;
-        d += lit;
+        if (lit <= 64 && s + lit <= wend) {
+            // a short run inside the window: one predicated copy
+            const std::uint32_t o = s - wbase;
+            if (std::uint32_t(lane) < lit) {
+                const std::uint8_t v = win[o + lane];
+                out[d + lane] = v;
+                ring[(d + lane) & mask] = v;
+            }
+            s += lit;
+            d += lit;
+            lit = 0;
+        }
+        while (lit > 0) {
+            if (s >= wend || s < wbase) refill(s);
+            const std::uint32_t n = min(lit, wend - s), o = s - wbase;
+            for (std::uint32_t i = lane; i < n; i += 64) {
+                const std::uint8_t v = win[o + i];
+                out[d + i] = v;
+                ring[(d + i) & mask] = v;
+            }
+            s += n;
+            d += n;
+            lit -= n;
+        }
         if (s == clen) break;
         if (clen - s < 2) return flag_error(3);
         const std::uint32_t off = byte(s) | byte(s + 1) << 8;
@@ -160,17 +221,45 @@ __global__ __launch_bounds__(64 * kWaves) void k_lz4_streams(const Stream* __res
         }
         m += 4;
         if (m > dlen - d) return flag_error(3);
-        // the source bytes [d - off, d - off + min(m, off)) were written by
-        // this wave's earlier copies: make them visible to its loads
-        if (d - off + min(m, off) > visible) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            visible = d;
+        if (m <= 64 && off >= m && off < std::uint32_t(kRing)) {
+            // a short match not overlapping itself: one predicated copy
+            if (std::uint32_t(lane) < m) {
+                const std::uint8_t v = ring[(d - off + lane) & mask];
+                out[d + lane] = v;
+                ring[(d + lane) & mask] = v;
+            }
+            d += m;
+        } else if (off < std::uint32_t(kRing)) {
+            while (m > 0) {
+                const std::uint32_t seg = min(m, std::uint32_t(kRing) - off), from = d - off;
+                // lane i reads byte from + (i mod off); r steps by 64 mod off
+                // (divisions only for offsets below 64)
+                std::uint32_t r = lane, step = 64;
+                if (off <= 64) {
+                    r = lane % off;
+                    step = 64 % off;
+                }
+                for (std::uint32_t i = lane; i < seg; i += 64) {
+                    const std::uint8_t v = ring[(from + r) & mask];
+                    out[d + i] = v;
+                    ring[(d + i) & mask] = v;
+                    r += step;
+                    if (r >= off) r -= off;
+                }
+                d += seg;
+                m -= seg;
+            }
+        } else {
+            __builtin_amdgcn_s_waitcnt(0xCF7F);  // vmcnt(63), expcnt and lgkmcnt unbounded
+            const std::uint32_t from = d - off;
+            for (std::uint32_t i = lane; i < m; i += 64) {
+                const std::uint8_t v =
+                    __hip_atomic_load(out + from + (i < off ? i : i % off), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                out[d + i] = v;
+                ring[(d + i) & mask] = v;
+            }
+            d += m;
         }
-        const std::uint32_t from = d - off;
-        for (std::uint32_t i = lane; i < m; i += 64) out[d + i] = out[from + (i < off ? i : i % off)];
-        d += m;
     }
     if (d != dlen) flag_error(3);
 }
@@ -275,30 +364,38 @@ std::size_t scratch_of(const std::uint8_t* src, std::size_t len) {
     return h.nbytes;
 }
 
-int decode_batch(const RangBloscChunk* chunks, std::size_t n, hipStream_t st) {
+// `resident` (optional): each chunk's bytes already in device memory (the
+// host copy in `chunks` is read for the plan only).
+int decode_batch(const RangBloscChunk* chunks, std::size_t n, const void* const* resident, hipStream_t st) {
     std::lock_guard<std::mutex> lock(g_pool.mu);
     std::size_t total = 0, scratch = 0;
     std::vector<std::size_t> at(n);
     for (std::size_t i = 0; i < n; ++i) {
-        if (!chunks[i].src || (!chunks[i].dst && chunks[i].dst_len)) refuse("null chunk pointer");
+        if (!chunks[i].src || (!chunks[i].dst && chunks[i].dst_len) || (resident && !resident[i]))
+            refuse("null chunk pointer");
         at[i] = total;
         total += (chunks[i].len + 15) / 16 * 16;
         scratch += scratch_of(chunks[i].src, chunks[i].len);
     }
-    std::uint8_t* dsrc = static_cast<std::uint8_t*>(g_pool.get(0, total));
+    std::uint8_t* dsrc = resident ? nullptr : static_cast<std::uint8_t*>(g_pool.get(0, total));
     std::uint8_t* dscr = static_cast<std::uint8_t*>(g_pool.get(1, scratch));
     std::vector<Stream> streams;
     std::vector<Shuffled> shuffled;
     std::size_t used = 0;
-    for (std::size_t i = 0; i < n; ++i)
-        plan_chunk(chunks[i].src, chunks[i].len, std::uint64_t(dsrc + at[i]), std::uint64_t(chunks[i].dst),
-                   chunks[i].dst_len, std::uint64_t(dscr), streams, shuffled, used);
-    // one staged upload of every chunk's bytes
-    std::vector<std::uint8_t> host(total);
-    for (std::size_t i = 0; i < n; ++i) std::memcpy(host.data() + at[i], chunks[i].src, chunks[i].len);
+    for (std::size_t i = 0; i < n; ++i) {
+        const std::uint64_t base = resident ? std::uint64_t(resident[i]) : std::uint64_t(dsrc + at[i]);
+        plan_chunk(chunks[i].src, chunks[i].len, base, std::uint64_t(chunks[i].dst), chunks[i].dst_len,
+                   std::uint64_t(dscr), streams, shuffled, used);
+    }
     Stream* dstreams = static_cast<Stream*>(g_pool.get(2, sizeof(Stream) * streams.size()));
     Shuffled* dshuf = static_cast<Shuffled*>(g_pool.get(3, sizeof(Shuffled) * shuffled.size()));
-    hipchk(hipMemcpyAsync(dsrc, host.data(), total, hipMemcpyHostToDevice, st), "upload");
+    // one staged upload of every chunk's bytes
+    std::vector<std::uint8_t> host;
+    if (!resident) {
+        host.resize(total);
+        for (std::size_t i = 0; i < n; ++i) std::memcpy(host.data() + at[i], chunks[i].src, chunks[i].len);
+        hipchk(hipMemcpyAsync(dsrc, host.data(), total, hipMemcpyHostToDevice, st), "upload");
+    }
     if (!streams.empty())
         hipchk(hipMemcpyAsync(dstreams, streams.data(), sizeof(Stream) * streams.size(), hipMemcpyHostToDevice, st),
                "upload");
@@ -310,7 +407,14 @@ int decode_batch(const RangBloscChunk* chunks, std::size_t n, hipStream_t st) {
            "status reset");
     if (!streams.empty()) {
         const int blocks = int((streams.size() + kWaves - 1) / kWaves);
-        k_lz4_streams<<<blocks, 64 * kWaves, 0, st>>>(dstreams, int(streams.size()));
+        static const bool lds_set = [] {
+            hipchk(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lz4_streams),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes),
+                   "hipFuncSetAttribute");
+            return true;
+        }();
+        (void)lds_set;
+        k_lz4_streams<<<blocks, 64 * kWaves, kLdsBytes, st>>>(dstreams, int(streams.size()));
         hipchk(hipGetLastError(), "k_lz4_streams");
     }
     if (!shuffled.empty()) {
@@ -382,7 +486,7 @@ int32_t rang_core_blosc_decode(const uint8_t* src, size_t len, uint8_t* out, siz
             ~Free() { (void)hipFree(p); }
         } guard{d};
         RangBloscChunk c{src, len, d, out_len};
-        decode_batch(&c, 1, nullptr);
+        decode_batch(&c, 1, nullptr, nullptr);
         hipchk(hipMemcpy(out, d, out_len, hipMemcpyDeviceToHost), "download");
         return 0;
     });
@@ -390,7 +494,13 @@ int32_t rang_core_blosc_decode(const uint8_t* src, size_t len, uint8_t* out, siz
 
 int32_t rang_amd_blosc_decode_batch(const RangBloscChunk* chunks, size_t n, void* stream) {
     if (!chunks && n) return 1;
-    return guarded([&] { return decode_batch(chunks, n, static_cast<hipStream_t>(stream)); });
+    return guarded([&] { return decode_batch(chunks, n, nullptr, static_cast<hipStream_t>(stream)); });
+}
+
+int32_t rang_amd_blosc_decode_resident(const RangBloscChunk* chunks, const void* const* dev_src, size_t n,
+                                       void* stream) {
+    if ((!chunks || !dev_src) && n) return 1;
+    return guarded([&] { return decode_batch(chunks, n, dev_src, static_cast<hipStream_t>(stream)); });
 }
 
 }  // extern "C"

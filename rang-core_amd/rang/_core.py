@@ -187,6 +187,7 @@ SIGNATURES = {
     "rang_core_blosc_header": ([_vp, _sz, ctypes.POINTER(_sz)], _i32),
     "rang_core_blosc_decode": ([_vp, _sz, _vp, _sz], _i32),
     "rang_amd_blosc_decode_batch": ([_vp, _sz, _vp], _i32),
+    "rang_amd_blosc_decode_resident": ([_vp, _vp, _sz, _vp], _i32),
     "rang_amd_blosc_last_error": ([_vp, _sz], _sz),
     "rang_amd_release_workspaces": ([], _i32),
     "rang_amd_wgridder_batch_vis2dirty_f32": ([ctypes.POINTER(BatchImage), _sz, _sz, _vp], _i32),

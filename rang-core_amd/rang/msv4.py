@@ -35,9 +35,10 @@ _NUMERIC = {"bool": "bool", "int8": "int8", "uint8": "uint8", "int16": "int16", 
             "float64": "float64", "complex64": "complex64", "complex128": "complex128"}
 
 
-def blosc_decode_device(chunks, outs):
+def blosc_decode_device(chunks, outs, resident=None):
     """Expand blosc-1 chunks (bytes) into device tensors `outs` (uint8 views
-    of each chunk's decoded length) in one upload and one launch."""
+    of each chunk's decoded length) in one upload and one launch; with
+    `resident` (device tensors holding the same bytes) nothing is uploaded."""
     import torch
 
     lib = _core.library()
@@ -47,7 +48,11 @@ def blosc_decode_device(chunks, outs):
         arr[i] = _core.BloscChunk(c.ctypes.data if c.size else None, c.size, o.data_ptr() if o.numel() else None,
                                   o.numel() * o.element_size())
     stream = torch.cuda.current_stream().cuda_stream
-    code = lib.rang_amd_blosc_decode_batch(arr, len(chunks), ctypes.c_void_p(stream))
+    if resident is not None:
+        ptrs = (ctypes.c_void_p * max(1, len(chunks)))(*[r.data_ptr() for r in resident])
+        code = lib.rang_amd_blosc_decode_resident(arr, ptrs, len(chunks), ctypes.c_void_p(stream))
+    else:
+        code = lib.rang_amd_blosc_decode_batch(arr, len(chunks), ctypes.c_void_p(stream))
     if code:
         n = lib.rang_amd_blosc_last_error(None, 0)
         buf = ctypes.create_string_buffer(n + 1)
