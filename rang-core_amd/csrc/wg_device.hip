@@ -1763,8 +1763,9 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
 
 // v2d: inverse transform along u of C image columns of one plane, then the
 // projection Re(g screen) of the image rows (wgridder.rs:2342-2397), one term
-// per plane and pixel.
-template <class T, int NT, int MAXTOT>
+// per plane and pixel. SCM: tmp column-major ([P][ny][nu], each column
+// contiguous), else row-major ([P][nu][ny]).
+template <class T, int NT, int MAXTOT, bool SCM = false>
 __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
                                                           T* __restrict__ proj, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
@@ -1779,10 +1780,16 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
     const cx<T>* src = tmp + (long long)p * nu * ny;
     const unsigned mnc = nc > 1 ? unsigned(((1ull << 32) + nc - 1) / nc) : 0u;
     // nu * nc <= MAXTOT
-    staged_fill<T, NT, MAXTOT>(buf, src, nu * nc, [&](int e) {
-        const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
-        return std::pair<long long, int>((long long)u * ny + j0 + c, c * nu + u);
-    });
+    if constexpr (SCM) {
+        staged_fill<T, NT, MAXTOT>(buf, src, nu * nc, [&](int e) {
+            return std::pair<long long, int>((long long)j0 * nu + e, e);
+        });
+    } else {
+        staged_fill<T, NT, MAXTOT>(buf, src, nu * nc, [&](int e) {
+            const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
+            return std::pair<long long, int>((long long)u * ny + j0 + c, c * nu + u);
+        });
+    }
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, +1);
     const double w0 = g.wmin + double(p) * g.dw;
@@ -1918,9 +1925,75 @@ __global__ __launch_bounds__(256) void k_correct_cm(const T* __restrict__ dirty,
     }
 }
 
+// Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
+// LDS tiles (64 x 64 in single precision, 32 x 32 in double, a column of
+// padding against bank conflicts): both sides coalesced.
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int R,
+                                                   int C) {
+    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+    __shared__ cx<T> t[TS][TS + 1];
+    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
+    const long long base = (long long)blockIdx.z * R * C;
+    for (int k = threadIdx.x; k < TS * TS; k += 256) {
+        const int rr = k / TS, cc = k - rr * TS, r = r0 + rr, c = c0 + cc;
+        if (r < R && c < C) t[rr][cc] = in[base + (long long)r * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TS * TS; k += 256) {
+        const int cc = k / TS, rr = k - cc * TS, r = r0 + rr, c = c0 + cc;
+        if (r < R && c < C) out[base + (long long)c * R + r] = t[rr][cc];
+    }
+}
+
+// The same for single precision with even R and C: each lane moves two
+// neighbouring values (16 bytes) per access on both sides.
+__global__ __launch_bounds__(256) void k_transpose_pairs(const cx<float>* __restrict__ in, cx<float>* __restrict__ out,
+                                                         int R, int C) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int TS = 64;
+    __shared__ cx<float> t[TS][TS + 1];
+    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
+    const long long base = (long long)blockIdx.z * R * C;
+    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
+        const int rr = k / (TS / 2), cp = 2 * (k - rr * (TS / 2)), r = r0 + rr, c = c0 + cp;
+        if (r < R && c < C) {
+            const f4 v = *reinterpret_cast<const f4*>(in + base + (long long)r * C + c);
+            t[rr][cp] = cx<float>{v.x, v.y};
+            t[rr][cp + 1] = cx<float>{v.z, v.w};
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
+        const int cc = k / (TS / 2), rp = 2 * (k - cc * (TS / 2)), r = r0 + rp, c = c0 + cc;
+        if (r < R && c < C) {
+            const cx<float> a = t[rp][cc], b = t[rp + 1][cc];
+            *reinterpret_cast<f4*>(out + base + (long long)c * R + r) = f4{a.re, a.im, b.re, b.im};
+        }
+    }
+}
+
+template <class T>
+void transpose_planes(const cx<T>* in, cx<T>* out, int P, int R, int C, hipStream_t st) {
+    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+    if (P > 65535) throw DevError{"too many planes for the transpose grid"};
+    const dim3 grid(unsigned((C + TS - 1) / TS), unsigned((R + TS - 1) / TS), unsigned(P));
+    if constexpr (sizeof(T) == 4) {
+        if (R % 2 == 0 && C % 2 == 0) {
+            k_transpose_pairs<<<grid, 256, 0, st>>>(in, out, R, C);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+    }
+    k_transpose<T><<<grid, 256, 0, st>>>(in, out, R, C);
+    HIPCHK(hipGetLastError());
+}
+
 // d2v: C image columns times the plane's w-screen placed on the grid rows,
-// forward transform along u (wgridder.rs:2931-2967).
-template <class T, int NT, int MAXTOT>
+// forward transform along u (wgridder.rs:2931-2967). TCM: the output
+// column-major ([P][ny][nu], each column's transform written contiguously,
+// transposed afterwards), else row-major ([P][nu][ny]).
+template <class T, int NT, int MAXTOT, bool TCM = false>
 __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
                                                           const cx<T>* __restrict__ tw, Geo g,
@@ -1983,9 +2056,13 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
     __syncthreads();
     fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, -1);
     cx<T>* dst = tmp + (long long)p * nu * ny;
-    for (int e = threadIdx.x; e < nu * nc; e += NT) {
-        const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
-        dst[(long long)u * ny + j0 + c] = buf[c * nu + u];
+    if constexpr (TCM) {
+        for (int e = threadIdx.x; e < nu * nc; e += NT) dst[(long long)j0 * nu + e] = buf[e];
+    } else {
+        for (int e = threadIdx.x; e < nu * nc; e += NT) {
+            const int u = int(udiv(unsigned(e), unsigned(nc), mnc)), c = e - u * nc;
+            dst[(long long)u * ny + j0 + c] = buf[c * nu + u];
+        }
     }
 }
 
@@ -2759,6 +2836,16 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
+// Column transforms on whole columns with a tiled transpose beside them,
+// unless RANG_FFT_TCM is 0 (then the columns are read or written strided).
+bool fft_tcm() {
+    static const bool on = [] {
+        const char* e = std::getenv("RANG_FFT_TCM");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // Gridding on the matrix cores unless RANG_GRID_MFMA is 0.
 bool mfma_enabled() {
     static const bool on = [] {
@@ -3442,11 +3529,24 @@ void grid_finish(Workspace& ws, const Geo& g, const GridPrep<T>& gp, int nout, T
             long_cols<T, 2>(tmp, P, g.ny, g.lu1, g.lu2, lt.u1, lt.u2, lt.tu1, lt.tu2, lt.tun, +1, proj, g,
                             tb.screen, st);
         } else {
+            // the grid of this output is consumed: it takes the columns of
+            // tmp, transposed, so that the u transform reads them whole
+            const bool scm = fft_tcm();
+            if (scm) {
+                transpose_planes<T>(tmp, gsrc, P, g.nu, g.ny, st);
+                timer.mark("transpose");
+            }
             fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
                 constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
                 const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-                set_lds(k_cols_inv<T, NT, MT>, lds_u);
-                k_cols_inv<T, NT, MT><<<blocks_of((long long)(g.ny + C - 1) / C * P, NT), NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+                const unsigned nb = unsigned(blocks_of((long long)(g.ny + C - 1) / C * P, NT));
+                if (scm) {
+                    set_lds(k_cols_inv<T, NT, MT, true>, lds_u);
+                    k_cols_inv<T, NT, MT, true><<<nb, NT, lds_u, st>>>(gsrc, proj, tb.du, tb.twu, g, tb.screen, C);
+                } else {
+                    set_lds(k_cols_inv<T, NT, MT>, lds_u);
+                    k_cols_inv<T, NT, MT><<<nb, NT, lds_u, st>>>(tmp, proj, tb.du, tb.twu, g, tb.screen, C);
+                }
             });
             HIPCHK(hipGetLastError());
         }
@@ -3933,6 +4033,11 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
     cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
     const int C = col_block(g.nu);
+    // when both transforms stay in LDS, the u transform writes whole
+    // columns into the grid (free until the v transform fills it) and a
+    // tiled transpose lays them out as rows (RANG_FFT_TCM=0: strided writes
+    // straight into tmp)
+    const bool tcm = !g.lu1 && !g.lv1 && fft_tcm();
     if (g.lu1) {
         k_place_cols<T><<<blocks_of((long long)P * g.nu, 256), 256, 0, st>>>(img, tmp, g, tb.screen);
         HIPCHK(hipGetLastError());
@@ -3942,12 +4047,23 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
         fft_dispatch<T>(g.nu * C, [&](auto nt, auto mt) {
             constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
             const std::size_t lds_u = sizeof(cx<T>) * g.nu * C;
-            set_lds(k_cols_fwd<T, NT, MT>, lds_u);
-            k_cols_fwd<T, NT, MT><<<blocks_of((long long)(g.ny + C - 1) / C * P, NT), NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+            const unsigned nb = unsigned(blocks_of((long long)(g.ny + C - 1) / C * P, NT));
+            if (tcm) {
+                // the columns into the (not yet used) grid, then transposed into tmp
+                set_lds(k_cols_fwd<T, NT, MT, true>, lds_u);
+                k_cols_fwd<T, NT, MT, true><<<nb, NT, lds_u, st>>>(img, grid, tb.du, tb.twu, g, tb.screen, C);
+            } else {
+                set_lds(k_cols_fwd<T, NT, MT>, lds_u);
+                k_cols_fwd<T, NT, MT><<<nb, NT, lds_u, st>>>(img, tmp, tb.du, tb.twu, g, tb.screen, C);
+            }
         });
         HIPCHK(hipGetLastError());
+        if (tcm) {
+            timer.mark("fft_u");
+            transpose_planes<T>(grid, tmp, P, g.ny, g.nu, st);
+        }
     }
-    timer.mark("fft_u");
+    timer.mark(tcm ? "transpose" : "fft_u");
     if (g.lv1) {
         const long long rows = (long long)P * g.nu;
         k_place_rows<T><<<blocks_of(rows, 256), 256, 0, st>>>(tmp, grid, g);
@@ -3957,8 +4073,9 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
         fft_dispatch<T>(g.nv, [&](auto nt, auto mt) {
             constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
             const std::size_t lds_v = sizeof(cx<T>) * g.nv;
+            const unsigned nb = unsigned(blocks_of((long long)P * g.nu, NT));
             set_lds(k_rows_fwd<T, NT, MT>, lds_v);
-            k_rows_fwd<T, NT, MT><<<blocks_of((long long)P * g.nu, NT), NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g);
+            k_rows_fwd<T, NT, MT><<<nb, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g);
         });
         HIPCHK(hipGetLastError());
     }
