@@ -2836,14 +2836,14 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
-// Column transforms on whole columns with a tiled transpose beside them,
-// unless RANG_FFT_TCM is 0 (then the columns are read or written strided).
-bool fft_tcm() {
-    static const bool on = [] {
-        const char* e = std::getenv("RANG_FFT_TCM");
-        return !(e && e[0] == '0');
-    }();
-    return on;
+// Column transforms on whole columns with a tiled transpose beside them
+// when the planes' image columns take `bytes` >= 512 MiB (measured: at
+// 25 GB 41 -> 20 + 10 ms per direction; at 136 MB the strided form is
+// 0.07 ms faster per step), unless RANG_FFT_TCM is 0 (never) or 1 (always).
+bool fft_tcm(std::size_t bytes) {
+    const char* e = std::getenv("RANG_FFT_TCM");  // read per call: tests switch it
+    if (e && e[0]) return std::atoi(e) != 0;
+    return bytes >= (std::size_t(1) << 29);
 }
 
 // Gridding on the matrix cores unless RANG_GRID_MFMA is 0.
@@ -3531,7 +3531,7 @@ void grid_finish(Workspace& ws, const Geo& g, const GridPrep<T>& gp, int nout, T
         } else {
             // the grid of this output is consumed: it takes the columns of
             // tmp, transposed, so that the u transform reads them whole
-            const bool scm = fft_tcm();
+            const bool scm = fft_tcm(sizeof(cx<T>) * std::size_t(P) * g.nu * g.ny);
             if (scm) {
                 transpose_planes<T>(tmp, gsrc, P, g.nu, g.ny, st);
                 timer.mark("transpose");
@@ -4037,7 +4037,7 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     // columns into the grid (free until the v transform fills it) and a
     // tiled transpose lays them out as rows (RANG_FFT_TCM=0: strided writes
     // straight into tmp)
-    const bool tcm = !g.lu1 && !g.lv1 && fft_tcm();
+    const bool tcm = !g.lu1 && !g.lv1 && fft_tcm(sizeof(cx<T>) * std::size_t(P) * g.nu * g.ny);
     if (g.lu1) {
         k_place_cols<T><<<blocks_of((long long)P * g.nu, 256), 256, 0, st>>>(img, tmp, g, tb.screen);
         HIPCHK(hipGetLastError());

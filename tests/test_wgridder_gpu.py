@@ -314,6 +314,30 @@ def test_two_pass_transforms_match_oracle(single, monkeypatch):
         assert rel_linf(d, ref) < TOL_ORACLE[single]
 
 
+@pytest.mark.parametrize("single", [True, False])
+@pytest.mark.parametrize("mode", ["0", "1"])
+def test_column_transforms_strided_and_transposed(single, mode, monkeypatch):
+    # The column transforms read or write whole columns with a tiled
+    # transpose beside them (the default for large planes), or strided
+    # columns (small planes); both forms in both precisions (the paired
+    # single-precision transpose and the plain one), dirty and PSF in one
+    # pass.
+    monkeypatch.setenv("RANG_FFT_TCM", mode)
+    for case in ({"nx": 64, "ny": 48, "nrow": 60}, {"nx": 40, "ny": 72, "nrow": 50}):
+        uvw, freq, vis, wgt, nx, ny, dx = ref_case(nx=case["nx"], ny=case["ny"], nrow=case["nrow"], seed=23)
+        eps = 1e-5 if single else 1e-9
+        got, ref, info, rinfo = v2d(uvw, freq, vis, wgt, nx, ny, dx, eps, single)
+        same_plan(info, rinfo)
+        assert rel_linf(got, ref) < TOL_ORACLE[single]
+        img = np.random.default_rng(5).standard_normal((nx, ny))
+        gv, rv, _, _ = d2v(uvw, freq, img, wgt, dx, eps, single)
+        assert rel_linf(gv, rv) < TOL_ORACLE[single]
+        d, p = W.dirty_psf(uvw=uvw, freq=freq, vis=vis.astype(np.complex64 if single else np.complex128),
+                           wgt=wgt.astype(np.float32 if single else np.float64), npix_x=nx, npix_y=ny,
+                           pixsize_x=dx, pixsize_y=dx, epsilon=eps, single=single)
+        assert rel_linf(d, ref) < TOL_ORACLE[single]
+
+
 def test_cells_no_visibility_reaches_are_cleared():
     """A call after a denser one: the grid's workspace still holds the first
     call's planes, and the (tile, plane group)s that none of the second
