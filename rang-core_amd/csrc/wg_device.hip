@@ -164,6 +164,7 @@ enum Slot : int {
     kSlotScalars,  // device scalars of a call (sums)
     kSlotItems2,   // degridding items of a prepared plan (gridding keeps kSlotItems)
     kSlotItemOff2,
+    kSlotOcc,  // per (plane group, tile): whether any gridding item reaches it
     kSlotCount
 };
 
@@ -936,14 +937,15 @@ __global__ __launch_bounds__(256) void k_grid_items(Geo g, const std::uint32_t* 
 }
 
 // Cells of the (tile, plane group)s gridded by more than one item, zeroed
-// before those items add into them, and of those gridded by none (no
-// visibility reaches them).
+// before those items add into them, and (unless `skip_empty`: the row
+// transform then takes them as zero) of those gridded by none.
 template <class T, int NOUT>
 __global__ __launch_bounds__(256) void k_zero_split(Geo g, const std::uint32_t* __restrict__ item_off,
                                                     const std::uint32_t* __restrict__ order, int G,
-                                                    int ngroups, cx<T>* __restrict__ grid) {
+                                                    int ngroups, cx<T>* __restrict__ grid, int skip_empty) {
     const int tg = blockIdx.x;
-    if (item_off[tg + 1] - item_off[tg] == 1) return;
+    const std::uint32_t n = item_off[tg + 1] - item_off[tg];
+    if (n == 1 || (n == 0 && skip_empty)) return;
     const int rank = tg / ngroups, grp = tg - rank * ngroups;
     const int tile = int(order[rank]);
     const int tu = tile / g.ntv, tv = tile - tu * g.ntv;
@@ -965,6 +967,16 @@ __global__ __launch_bounds__(256) void k_zero_split(Geo g, const std::uint32_t* 
                 base[(long long)grow * g.nv] = cx<T>{T(0), T(0)};
             }
         }
+}
+
+// occ[group][tile] = 1 when a gridding item reaches the (tile, plane group).
+__global__ __launch_bounds__(256) void k_occupancy(const std::uint32_t* __restrict__ item_off,
+                                                   const std::uint32_t* __restrict__ order, int ntg, int ngroups,
+                                                   int ntiles, std::uint8_t* __restrict__ occ) {
+    const int tg = blockIdx.x * 256 + threadIdx.x;
+    if (tg >= ntg) return;
+    const int rank = tg / ngroups, grp = tg - rank * ngroups;
+    occ[(long long)grp * ntiles + order[rank]] = item_off[tg + 1] > item_off[tg] ? 1 : 0;
 }
 
 // The item `item` of a prefix array of per-unit item counts: the unit and
@@ -1732,23 +1744,82 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     return x * per + min(x, rem) + k;
 }
 
+// Which tiles of a grid row were gridded: with `occ` (per plane group and
+// tile, nonzero when a gridding item reached it) the row transform reads
+// only those tiles' cells and takes the others as zero, so cells no
+// visibility reaches are neither cleared before gridding nor read after.
+struct RowOcc {
+    const std::uint8_t* occ;  // null: every cell is read
+    int G;                    // planes per group
+    unsigned mtv;             // udiv multiplier for the tile side Tv
+};
+
 // v2d: inverse transform along v of one grid row, keeping the image columns
 // (wgridder.rs:2331-2338, 1745-1749).
 template <class T, int NT, int MAXTOT>
 __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
                                                           cx<T>* __restrict__ tmp, FftDesc d,
-                                                          const cx<T>* __restrict__ tw, int ny) {
+                                                          const cx<T>* __restrict__ tw, int ny, Geo g, RowOcc ro) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nv = d.n;
     const long long row = blockIdx.x;
     const cx<T>* src = grid + row * nv;
-    staged_fill<T, NT, MAXTOT>(buf, src, nv, [](int e) { return std::pair<long long, int>(e, e); });
-    __syncthreads();
+    int nz = 0;
+    if (ro.occ) {
+        // the row's tiles: grid row u is tile row (u - nu/2) mod nu / Tu of
+        // plane p, column v tile column (v - nv/2) mod nv / Tv
+        const int p = int(row / g.nu), u = int(row - (long long)p * g.nu);
+        int us = u - g.nu / 2;
+        if (us < 0) us += g.nu;
+        const std::uint8_t* orow = ro.occ + ((long long)(p / ro.G) * g.ntu + us / g.Tu) * g.ntv;
+        // thread t takes the shifted columns vs = t + k NT (physical column
+        // vs + nv/2 mod nv): its tile column is t / Tv + k NT / Tv when Tv
+        // divides NT, one division per thread
+        constexpr int PER = (MAXTOT + NT - 1) / NT;
+        const int half = nv / 2;
+        const bool even = NT % g.Tv == 0;
+        const int t0 = int(udiv(threadIdx.x, unsigned(g.Tv), ro.mtv)), tstep = NT / g.Tv;
+        // every occupancy byte first, then the loads of the occupied cells
+        // (no wait between them)
+        std::uint8_t oc[PER];
+        if (even) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int vs = int(threadIdx.x) + k * NT;
+                const std::uint8_t o = orow[min(t0 + k * tstep, g.ntv - 1)];
+                oc[k] = vs < nv ? o : std::uint8_t(0);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int vs = int(threadIdx.x) + k * NT;
+                const std::uint8_t o = orow[min(int(udiv(unsigned(vs), unsigned(g.Tv), ro.mtv)), g.ntv - 1)];
+                oc[k] = vs < nv ? o : std::uint8_t(0);
+            }
+        }
+        cx<T> r[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int vs = int(threadIdx.x) + k * NT;
+            const int e = vs + half < nv ? vs + half : vs + half - nv;
+            r[k] = oc[k] ? src[e] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int vs = int(threadIdx.x) + k * NT;
+            if (vs < nv) {
+                buf[vs + half < nv ? vs + half : vs + half - nv] = r[k];
+                nz |= (r[k].re != T(0) || r[k].im != T(0)) ? 1 : 0;
+            }
+        }
+    } else {
+        staged_fill<T, NT, MAXTOT>(buf, src, nv, [](int e) { return std::pair<long long, int>(e, e); });
+        __syncthreads();
+        for (int e = threadIdx.x; e < nv; e += NT) nz |= (buf[e].re != T(0) || buf[e].im != T(0)) ? 1 : 0;
+    }
     // a row no visibility reached (beyond the coverage on this plane)
     // transforms to zero: write the zeros, skip the passes
-    int nz = 0;
-    for (int e = threadIdx.x; e < nv; e += NT) nz |= (buf[e].re != T(0) || buf[e].im != T(0)) ? 1 : 0;
     if (!__syncthreads_or(nz)) {
         for (int j = threadIdx.x; j < ny; j += NT) tmp[row * ny + j] = cx<T>{T(0), T(0)};
         return;
@@ -2836,6 +2907,12 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
+// The row transform reads only the gridded tiles unless RANG_GRID_OCC is 0.
+bool grid_occ() {
+    const char* e = std::getenv("RANG_GRID_OCC");
+    return !(e && e[0] == '0');
+}
+
 // Column transforms on whole columns with a tiled transpose beside them
 // when the planes' image columns take `bytes` >= 512 MiB (measured: at
 // 25 GB 41 -> 20 + 10 ms per direction; at 136 MB the strided form is
@@ -3440,6 +3517,7 @@ struct GridPrep {
     std::uint32_t* item_off = nullptr;
     long long ub = 0;  // workgroups of the gridding launch
     cx<T>* grid = nullptr;
+    const std::uint8_t* occ = nullptr;  // tiles gridded, for the row transform (null: all cells cleared)
 };
 
 // Tables, the grid (its cells reached by several items cleared) and the
@@ -3479,10 +3557,20 @@ void grid_prepare(Workspace& ws, const Geometry& G, const Geo& g, const std::uin
         HIPCHK(hipGetLastError());
         exclusive_scan(ws, items, gp.item_off, ntg + 1, st);
     }
+    // the in-LDS row transform reads only the tiles an item reached
+    // (RANG_GRID_OCC=0: every cell cleared and read)
+    gp.occ = nullptr;
+    if (!g.lv1 && grid_occ()) {
+        std::uint8_t* occ = static_cast<std::uint8_t*>(ws.get(kSlotOcc, std::size_t(ntg)));
+        k_occupancy<<<(ntg + 255) / 256, 256, 0, st>>>(gp.item_off, gp.tb.order, ntg, gp.ngroups, g.ntu * g.ntv, occ);
+        HIPCHK(hipGetLastError());
+        gp.occ = occ;
+    }
+    const int skip = gp.occ ? 1 : 0;
     if (nout == 2)
-        k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid);
+        k_zero_split<T, 2><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid, skip);
     else
-        k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid);
+        k_zero_split<T, 1><<<ntg, 256, 0, st>>>(g, gp.item_off, gp.tb.order, gp.G_planes, gp.ngroups, gp.grid, skip);
     HIPCHK(hipGetLastError());
     if (cache && cache->valid) {
         gp.ub = cache->ub;
@@ -3520,7 +3608,9 @@ void grid_finish(Workspace& ws, const Geo& g, const GridPrep<T>& gp, int nout, T
                 constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
                 const std::size_t lds_v = sizeof(cx<T>) * g.nv;
                 set_lds(k_rows_inv<T, NT, MT>, lds_v);
-                k_rows_inv<T, NT, MT><<<blocks_of((long long)P * g.nu, NT), NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv, g.ny);
+                const RowOcc ro{gp.occ, gp.G_planes, fft_magic(unsigned(g.Tv))};
+                k_rows_inv<T, NT, MT><<<blocks_of((long long)P * g.nu, NT), NT, lds_v, st>>>(gsrc, tmp, tb.dv, tb.twv,
+                                                                                             g.ny, g, ro);
             });
             HIPCHK(hipGetLastError());
         }
