@@ -165,6 +165,7 @@ enum Slot : int {
     kSlotItems2,   // degridding items of a prepared plan (gridding keeps kSlotItems)
     kSlotItemOff2,
     kSlotOcc,  // per (plane group, tile): whether any gridding item reaches it
+    kSlotRowNeed,  // per grid row: whether a degridding item reads it
     kSlotCount
 };
 
@@ -2142,12 +2143,15 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
 template <class T, int NT, int MAXTOT>
 __global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
                                                           cx<T>* __restrict__ grid, FftDesc d,
-                                                          const cx<T>* __restrict__ tw, Geo g) {
+                                                          const cx<T>* __restrict__ tw, Geo g,
+                                                          const std::uint8_t* __restrict__ need) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nv = d.n, ny = g.ny;
     const long long row = blockIdx.x;
     const long long p = row / g.nu;
+    // a row no degridding item reads is neither transformed nor written
+    if (need && !need[row - p * g.nu]) return;
     const long long src = p * g.nu + upos(g, int(row - p * g.nu));
     // the image row in flight while the buffer is cleared
     constexpr int PER = (MAXTOT + NT - 1) / NT;
@@ -2372,6 +2376,25 @@ __global__ __launch_bounds__(256) void k_degrid_items(Geo g, const std::uint32_t
     const int tile = int(order[r]);
     const std::uint32_t cnt = off[(long long)(tile + 1) * 4 * g.nb] - off[(long long)tile * 4 * g.nb];
     nitems[r] = (cnt + rc - 1) / rc;
+}
+
+// need[u] = 1 for the grid rows a degridding item reads: the rows of its
+// tile and halo (taken S rows wider on both sides), for every tile with
+// items; rows no visibility reaches are left 0 and the forward row
+// transform skips them.
+__global__ __launch_bounds__(256) void k_row_need(Geo g, const std::uint32_t* __restrict__ item_off,
+                                                  const std::uint32_t* __restrict__ order,
+                                                  std::uint8_t* __restrict__ need) {
+    const int nt = g.ntu * g.ntv;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nt || item_off[r + 1] == item_off[r]) return;
+    const int tu = int(order[r]) / g.ntv;
+    for (int us = tu * g.Tu - g.supp; us < (tu + 1) * g.Tu + g.supp; ++us) {
+        int u = us + g.nu / 2;
+        u %= g.nu;
+        if (u < 0) u += g.nu;
+        need[u] = 1;
+    }
 }
 
 // d2v: degridding (wgridder.rs:2793-2901) of one item: per visibility of the
@@ -2907,7 +2930,8 @@ bool long_split(int n, int lim, int& n1, int& n2) {
 // Geometry and tiling into the kernels' Geo. Gridding keeps a plane of the
 // tile's accumulators within 40 KiB; degridding prefers tiles whose halo of
 // every plane fits the staged LDS limit.
-// The row transform reads only the gridded tiles unless RANG_GRID_OCC is 0.
+// The row transforms read only the gridded tiles (vis2dirty) and write only
+// the rows degridding reads (dirty2vis) unless RANG_GRID_OCC is 0.
 bool grid_occ() {
     const char* e = std::getenv("RANG_GRID_OCC");
     return !(e && e[0] == '0');
@@ -4122,6 +4146,39 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
     HIPCHK(hipGetLastError());
     cx<T>* tmp = static_cast<cx<T>*>(ws.get(kSlotTmp, sizeof(cx<T>) * (long long)P * g.nu * g.ny));
     cx<T>* grid = static_cast<cx<T>*>(ws.get(kSlotGrid, sizeof(cx<T>) * plane * P));
+    // the degridding items (before the transforms: the forward row
+    // transform writes only the rows they read)
+    const int nt = g.ntu * g.ntv;
+    // a prepared plan's degridding items stay in slots of their own
+    std::uint32_t* items = static_cast<std::uint32_t*>(
+        ws.get(cache ? kSlotItems2 : kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
+    std::uint32_t* item_off = static_cast<std::uint32_t*>(
+        ws.get(cache ? kSlotItemOff2 : kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
+    const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
+    const bool staged = dl <= degrid_lds_max<T>();
+    const std::uint32_t rc = staged ? degrid_chunk_staged<T>() : kDegridChunk;
+    long long ub;
+    if (cache && cache->valid) {
+        ub = cache->ub;
+    } else {
+        k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, rc, items);
+        HIPCHK(hipGetLastError());
+        exclusive_scan(ws, items, item_off, nt + 1, st);
+        ub = read_u32(ws, item_off + nt, st);
+        if (cache) {
+            cache->ub = ub;
+            cache->valid = true;
+        }
+    }
+    if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
+    std::uint8_t* need = nullptr;
+    if (!g.lv1 && grid_occ()) {
+        const std::size_t nb4 = (std::size_t(g.nu) + 3) / 4 * 4;
+        need = static_cast<std::uint8_t*>(ws.get(kSlotRowNeed, nb4));
+        fill_zero(need, nb4, st);
+        k_row_need<<<(nt + 255) / 256, 256, 0, st>>>(g, item_off, tb.order, need);
+        HIPCHK(hipGetLastError());
+    }
     const int C = col_block(g.nu);
     // when both transforms stay in LDS, the u transform writes whole
     // columns into the grid (free until the v transform fills it) and a
@@ -4165,34 +4222,11 @@ void degrid_pipeline(Workspace& ws, const Geometry& G, const Geo& g, const Rec<T
             const std::size_t lds_v = sizeof(cx<T>) * g.nv;
             const unsigned nb = unsigned(blocks_of((long long)P * g.nu, NT));
             set_lds(k_rows_fwd<T, NT, MT>, lds_v);
-            k_rows_fwd<T, NT, MT><<<nb, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g);
+            k_rows_fwd<T, NT, MT><<<nb, NT, lds_v, st>>>(tmp, grid, tb.dv, tb.twv, g, need);
         });
         HIPCHK(hipGetLastError());
     }
     timer.mark("fft_v");
-    const int nt = g.ntu * g.ntv;
-    // a prepared plan's degridding items stay in slots of their own
-    std::uint32_t* items = static_cast<std::uint32_t*>(
-        ws.get(cache ? kSlotItems2 : kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
-    std::uint32_t* item_off = static_cast<std::uint32_t*>(
-        ws.get(cache ? kSlotItemOff2 : kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
-    const std::size_t dl = degrid_lds<T>(g.Tu, g.Tv, g.supp, P);
-    const bool staged = dl <= degrid_lds_max<T>();
-    const std::uint32_t rc = staged ? degrid_chunk_staged<T>() : kDegridChunk;
-    long long ub;
-    if (cache && cache->valid) {
-        ub = cache->ub;
-    } else {
-        k_degrid_items<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, tb.order, rc, items);
-        HIPCHK(hipGetLastError());
-        exclusive_scan(ws, items, item_off, nt + 1, st);
-        ub = read_u32(ws, item_off + nt, st);
-        if (cache) {
-            cache->ub = ub;
-            cache->valid = true;
-        }
-    }
-    if (ub >= (1LL << 31)) throw DevError{"too many degridding items"};
     DegridOut<T> o = outs;
     if (o.mode == 2) o.part = static_cast<double*>(ws.get(kSlotRed, sizeof(double) * std::max<long long>(ub, 1)));
     if (ub == 0) {
