@@ -316,13 +316,17 @@ def test_two_pass_transforms_match_oracle(single, monkeypatch):
 
 @pytest.mark.parametrize("single", [True, False])
 @pytest.mark.parametrize("mode", ["0", "1"])
-def test_column_transforms_strided_and_transposed(single, mode, monkeypatch):
+@pytest.mark.parametrize("occ", ["0", "1"])
+def test_column_transforms_strided_and_transposed(single, mode, occ, monkeypatch):
     # The column transforms read or write whole columns with a tiled
     # transpose beside them (the default for large planes), or strided
     # columns (small planes); both forms in both precisions (the paired
     # single-precision transpose and the plain one), dirty and PSF in one
-    # pass.
+    # pass. With `occ` the row transforms read only the tiles an item
+    # reached and write only the rows degridding reads (the default for
+    # large planes).
     monkeypatch.setenv("RANG_FFT_TCM", mode)
+    monkeypatch.setenv("RANG_GRID_OCC", occ)
     for case in ({"nx": 64, "ny": 48, "nrow": 60}, {"nx": 40, "ny": 72, "nrow": 50}):
         uvw, freq, vis, wgt, nx, ny, dx = ref_case(nx=case["nx"], ny=case["ny"], nrow=case["nrow"], seed=23)
         eps = 1e-5 if single else 1e-9
@@ -338,11 +342,14 @@ def test_column_transforms_strided_and_transposed(single, mode, monkeypatch):
         assert rel_linf(d, ref) < TOL_ORACLE[single]
 
 
-def test_cells_no_visibility_reaches_are_cleared():
+@pytest.mark.parametrize("occ", ["0", "1"])
+def test_cells_no_visibility_reaches_are_cleared(occ, monkeypatch):
     """A call after a denser one: the grid's workspace still holds the first
     call's planes, and the (tile, plane group)s that none of the second
-    call's visibilities reach are cleared rather than gridded, so nothing of
-    the first call may show in the second's image."""
+    call's visibilities reach are cleared rather than gridded (or, with
+    `occ`, skipped by the row transform), so nothing of the first call may
+    show in the second's image."""
+    monkeypatch.setenv("RANG_GRID_OCC", occ)
     uvw, freq, vis, wgt, nx, ny, dx = ref_case(nx=64, ny=64, nrow=300, seed=21, nchan=4)
     keep = np.zeros(vis.shape, dtype=np.uint8)
     keep[:12] = 1
