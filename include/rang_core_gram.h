@@ -77,6 +77,29 @@ int32_t rang_amd_wgridder_prepared_pcg_f32(RangPrepared *p, RangGram *m, const f
                                            float *x, int32_t iterations, double tolerance,
                                            int32_t *used, double *rel, void *stream);
 
+/* prox_solver::defect_correction (core/src/prox_solver.rs:778-875): major
+ * cycles of the L1-regularised problem (prior lambda * factor_i, x >= 0
+ * unless signed_) on the prepared plan's exact normal operator, each inner
+ * FISTA solve (settings `inner`; its gap tolerance is not used) on the cheap
+ * operator `cheap` with the exact operator's defect at the cycle's iterate
+ * moved into the data. dirty = Phi^H W y and q0 = 1/2 |y|^2_W as for
+ * rang_amd_wgridder_prepared_fista_*; x holds the start and receives the last
+ * judged iterate, gx (may be NULL) its exact G x; records (may be NULL) one
+ * per cycle, at most nrecords. The cheap operator's side must equal the
+ * plan's square image. */
+int32_t rang_amd_wgridder_prepared_defect_f64(RangPrepared *p, RangGram *cheap, const double *dirty,
+                                              double q0, const double *factors, int32_t signed_,
+                                              const RangFistaSettings *inner,
+                                              const RangOuterSchedule *schedule, double *x, double *gx,
+                                              RangOuterRecord *records, size_t nrecords,
+                                              RangDefectOutcome *out, void *stream);
+int32_t rang_amd_wgridder_prepared_defect_f32(RangPrepared *p, RangGram *cheap, const float *dirty,
+                                              double q0, const float *factors, int32_t signed_,
+                                              const RangFistaSettings *inner,
+                                              const RangOuterSchedule *schedule, float *x, float *gx,
+                                              RangOuterRecord *records, size_t nrecords,
+                                              RangDefectOutcome *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

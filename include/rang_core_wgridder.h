@@ -325,6 +325,39 @@ int32_t rang_amd_wgridder_prepared_fista_f32(RangPrepared *p, const float *dirty
                                              const RangFistaSettings *settings, float *x,
                                              float *gx, RangFistaRecord *records, size_t nrecords,
                                              RangFistaOutcome *out, void *stream);
+/* prox_solver::OuterSchedule (:760-772): at most `cycles` major cycles, stop
+ * when the exact optimality residual at `lambda` is <= kkt; a cycle solves at
+ * max(lambda, gamma * peak residual). */
+typedef struct RangOuterSchedule {
+    uint64_t cycles;
+    double kkt;
+    double lambda;
+    double gamma;
+} RangOuterSchedule;
+/* prox_solver::OuterRecord (:447-462): one major cycle, judged exactly. */
+typedef struct RangOuterRecord {
+    uint64_t outer;
+    double objective;
+    double misfit;
+    double gap;
+    double kkt;
+    uint64_t inner_iterations; /* inner iterations summed before this cycle */
+    double defect_sup;         /* max |G x - G~ x| */
+} RangOuterRecord;
+/* How defect_correction ended (prox_solver.rs:778-875): `converged` 1 for
+ * "converged", 0 for "cycles"; the inner iterations summed; the exact and
+ * cheap operator applications made. */
+typedef struct RangDefectOutcome {
+    uint64_t iterations;
+    int32_t converged;
+    int32_t reserved;
+    uint64_t cycles;
+    uint64_t restarts;
+    uint64_t negative_curvatures;
+    uint64_t exact_applies;
+    uint64_t cheap_applies;
+    RangOuterRecord last;
+} RangDefectOutcome;
 /* prox_solver::power_iteration (:886-908): the largest eigenvalue of the normal
  * operator from the reference's seeded start (xorshift64, :871-882), stopping
  * at a relative change <= tolerance; *used applications. */

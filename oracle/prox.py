@@ -9,6 +9,9 @@ It follows prox_solver.rs:
   * certificate       - :236-281 (misfit, penalty, duality gap, KKT residual)
   * solve             - :283-445 (FISTA with the linearity trick and gradient
                                   restart; forward-backward without momentum)
+  * defect_correction - :760-875 (major cycles: exact certificate, the cheap
+                                  operator's defect moved into the data, FISTA
+                                  on the cheap operator)
   * power_iteration   - :886-908
 
 The operator is any callable gram(x) -> G x on flat float64 arrays. Sums run
@@ -16,7 +19,8 @@ serially in float64 (the reference sums in parallel; both agree to rounding).
 Pinned by the reference's own known answers (tests/test_prox_oracle.py): the
 proximal map of prox_solver.rs:1066-1076, the dense problem of :1000-1063 with
 its gap, linearity, KKT, curvature and forward-backward checks of :1078-1133
-and :1241-1276, and the diagonal spectrum of :1278-1300.
+and :1241-1276, the diagonal spectrum of :1278-1300, and the defect
+correction from an operator 20 % off of :1301-1376.
 """
 
 import math
@@ -184,6 +188,57 @@ def solve(gram, dirty, q0, prior, x0, *, step, max_iterations=100, momentum=True
         y = x + next_beta * (x - x_prev)
         beta = next_beta
     raise AssertionError("the loop returns at max_iterations")
+
+
+def defect_correction(exact, cheap, dirty, q0, prior, x0, *, step, max_iterations, cycles, kkt, lam,
+                      gamma=0.0, momentum=True, restart=True, kkt_tolerance=None, step_tolerance=None,
+                      observer=None):
+    """Major cycles on a cheap operator with its error at x_j moved into the
+    data (prox_solver.rs:760-875): each cycle applies both operators at x_j,
+    judges x_j by the exact certificate at `lam`, and solves
+    min 1/2 <x, G~ x> - <d - (G - G~) x_j, x> + R(x) with FISTA on the cheap
+    operator from x_j at max(lam, gamma * peak residual), no gap test. Returns
+    a dict with x, gx (exact G x), last (objective, misfit, gap, kkt),
+    iterations (inner, summed), stop, restarts, negative_curvatures,
+    records (one per cycle), exact_applies and cheap_applies."""
+    dirty = np.asarray(dirty, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    out = dict(restarts=0, negative_curvatures=0, records=[], exact_applies=0, cheap_applies=0)
+    inner_iterations = 0
+    for j in range(cycles + 1):
+        g = np.asarray(exact(x), dtype=np.float64)
+        out["exact_applies"] += 1
+        prior.set_lambda(lam)
+        misfit, pen, gap, k_res = certificate(dirty, q0, x, g, prior)
+        g_cheap = np.asarray(cheap(x), dtype=np.float64)
+        out["cheap_applies"] += 1
+        c = g - g_cheap
+        defect_sup = float(np.max(np.abs(c))) if c.size else 0.0
+        corrected = dirty - c
+        record = dict(outer=j, objective=misfit + pen, misfit=misfit, gap=gap, kkt=k_res,
+                      inner_iterations=inner_iterations, defect_sup=defect_sup)
+        out["records"].append(record)
+        if observer is not None:
+            observer(record, x)
+        out["last"] = dict(objective=misfit + pen, misfit=misfit, gap=gap, kkt=k_res)
+        if k_res <= kkt or j == cycles:
+            out["stop"] = "converged" if k_res <= kkt else "cycles"
+            out["gx"] = g
+            out["x"] = x
+            out["iterations"] = inner_iterations
+            return out
+        r = dirty - g
+        peak = max(0.0, float(np.max(np.abs(r) if prior.signed else r))) if r.size else 0.0
+        prior.set_lambda(max(lam, gamma * peak))
+        solved = solve(cheap, corrected, q0, prior, x, step=step, max_iterations=max_iterations,
+                       momentum=momentum, restart=restart, gap_tolerance=None, kkt_tolerance=kkt_tolerance,
+                       step_tolerance=step_tolerance)
+        inner_iterations += solved["iterations"]
+        out["restarts"] += solved["restarts"]
+        out["negative_curvatures"] += solved["negative_curvatures"]
+        out["cheap_applies"] += solved["applies"]
+        x = solved["x"]
+    raise AssertionError("the loop returns at the last cycle")
 
 
 def power_iteration(gram, n2, seed, tolerance, max_iterations):

@@ -409,6 +409,38 @@ __global__ __launch_bounds__(NT) void k_ap_cols(cx<T>* __restrict__ tmp_all, int
     }
 }
 
+// The same on the transposed slice: column j of the half-transformed array
+// held as n contiguous values at tmpT[j * n] (rows n .. 2n - 1 zero), so
+// both the load and the store of a column are coalesced.
+template <class T, int NT, int MT>
+__global__ __launch_bounds__(NT) void k_ap_cols_t(cx<T>* __restrict__ tmpt_all, int n, const T* const* __restrict__ specs,
+                                                  int C, FftDesc d, const cx<T>* __restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(lds_raw);
+    const int big = 2 * n;
+    cx<T>* tt = tmpt_all + (long long)blockIdx.y * n * big;
+    const T* spec = specs[blockIdx.y];
+    const int j0 = blockIdx.x * C;
+    const int nc = min(C, big - j0);
+    for (int e = threadIdx.x; e < nc * big; e += NT) {
+        const int c = e / big, i = e - c * big;
+        buf[e] = i < n ? tt[(long long)(j0 + c) * n + i] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d, nc, tw, -1);
+    for (int e = threadIdx.x; e < nc * big; e += NT) {
+        const int c = e / big, i = e - c * big;
+        const T g = spec[(long long)(j0 + c) * big + i];
+        buf[e] = cx<T>{buf[e].re * g, buf[e].im * g};
+    }
+    __syncthreads();
+    fft_lds<T, NT, MT>(buf, d, nc, tw, +1);
+    for (int e = threadIdx.x; e < nc * big; e += NT) {
+        const int c = e / big, i = e - c * big;
+        if (i < n) tt[(long long)(j0 + c) * n + i] = buf[e];
+    }
+}
+
 // Row i of every node's slice transformed back; out = sum over nodes of
 // Re(y_k e_k) over the first n values, summed in registers and written once.
 template <class T, int NT, int MT>
@@ -510,6 +542,7 @@ struct GramImpl final : GramBase {
     cx<T>* tw = nullptr;
     cx<T>* tmp = nullptr;      // n x 2n for the spectra's set-up
     cx<T>* tmp_all = nullptr;  // one n x 2n slice per node for applications
+    cx<T>* tmpt_all = nullptr;  // the slices transposed (2n x n), for large images
     T** d_spec = nullptr;      // the spectra's device addresses
     double st[kGramStats] = {};
     std::mutex mu;
@@ -521,6 +554,7 @@ struct GramImpl final : GramBase {
         if (tw) (void)hipFree(tw);
         if (tmp) (void)hipFree(tmp);
         if (tmp_all) (void)hipFree(tmp_all);
+        if (tmpt_all) (void)hipFree(tmpt_all);
         if (d_spec) (void)hipFree(d_spec);
         if (cur != device) (void)hipSetDevice(cur);
     }
@@ -566,16 +600,46 @@ struct GramImpl final : GramBase {
     // Nodes per pass: their half-transformed slices within 2 GiB
     // (RANG_GRAM_PASS_BYTES lowers the budget, a test knob; it is read at an
     // operator's first application, which sizes the slices).
+    //
+    // A column pass reading the n x 2n slice in place touches one cell per
+    // row; once a slice outgrows the caches (256 MiB and more, n >= 4096 in
+    // single precision) the slices are transposed through LDS tiles around
+    // the column pass instead, which then reads and writes whole columns
+    // (RANG_GRAM_TRANSPOSE=0 or 1 forces either form, a test knob read with
+    // the pass budget).
     int pass_nodes = 0;
+    bool transposed = false;
+    static bool transpose_for(std::size_t slice) {
+        if (const char* e = std::getenv("RANG_GRAM_TRANSPOSE")) {
+            if (e[0] == '0') return false;
+            if (e[0] == '1') return true;
+        }
+        return slice >= (std::size_t(256) << 20);
+    }
     int nodes_per_pass() const {
         std::size_t budget = std::size_t(2) << 30;
         if (const char* e = std::getenv("RANG_GRAM_PASS_BYTES")) {
             const long long v = std::atoll(e);
             if (v > 0 && std::size_t(v) < budget) budget = std::size_t(v);
         }
-        const std::size_t slice = sizeof(cx<T>) * std::size_t(n) * 2 * n;
+        const std::size_t slice = sizeof(cx<T>) * std::size_t(n) * 2 * n * (transposed ? 2 : 1);
         const std::size_t k = std::max<std::size_t>(1, budget / slice);
         return int(std::min<std::size_t>(k, spectra.size()));
+    }
+
+    // [P][R][C] -> [P][C][R] through LDS tiles.
+    static void transpose(const cx<T>* in, cx<T>* out, int P, int R, int C, hipStream_t s) {
+        constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+        const dim3 grid(unsigned((C + TS - 1) / TS), unsigned((R + TS - 1) / TS), unsigned(P));
+        if constexpr (sizeof(T) == 4) {
+            if (R % 2 == 0 && C % 2 == 0) {
+                k_tile_transpose_pairs<T><<<grid, 256, 0, s>>>(in, out, R, C);
+                GCHK(hipGetLastError());
+                return;
+            }
+        }
+        k_tile_transpose<T><<<grid, 256, 0, s>>>(in, out, R, C);
+        GCHK(hipGetLastError());
     }
 
     void apply(const T* x, T* out, hipStream_t s) {
@@ -583,8 +647,10 @@ struct GramImpl final : GramBase {
         const int C = cols_per_block();
         const int K = int(spectra.size());
         if (!tmp_all) {
+            transposed = transpose_for(sizeof(cx<T>) * std::size_t(n) * big);
             pass_nodes = nodes_per_pass();
             GCHK(hipMalloc(&tmp_all, sizeof(cx<T>) * std::size_t(pass_nodes) * n * big));
+            if (transposed) GCHK(hipMalloc(&tmpt_all, sizeof(cx<T>) * std::size_t(pass_nodes) * n * big));
             GCHK(hipMalloc(&d_spec, sizeof(T*) * std::size_t(K)));
             GCHK(hipMemcpy(d_spec, spectra.data(), sizeof(T*) * std::size_t(K), hipMemcpyHostToDevice));
         }
@@ -600,14 +666,27 @@ struct GramImpl final : GramBase {
                 k_ap_rows_fwd<T, NT, MT><<<dim3(n, kn), NT, lds, s>>>(x, n, uw, wk0, dw, dx, dy, tmp_all, d, tw);
             });
             GCHK(hipGetLastError());
-            dispatch<T>(big * C, [&](auto nt, auto mt) {
-                constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
-                const std::size_t lds = sizeof(cx<T>) * big * C;
-                lds_attr(k_ap_cols<T, NT, MT>, lds);
-                k_ap_cols<T, NT, MT><<<dim3((big + C - 1) / C, kn), NT, lds, s>>>(tmp_all, n, d_spec + k0, C, d,
-                                                                                  tw);
-            });
-            GCHK(hipGetLastError());
+            if (transposed) {
+                transpose(tmp_all, tmpt_all, kn, n, big, s);
+                dispatch<T>(big * C, [&](auto nt, auto mt) {
+                    constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                    const std::size_t lds = sizeof(cx<T>) * big * C;
+                    lds_attr(k_ap_cols_t<T, NT, MT>, lds);
+                    k_ap_cols_t<T, NT, MT><<<dim3((big + C - 1) / C, kn), NT, lds, s>>>(tmpt_all, n, d_spec + k0, C,
+                                                                                        d, tw);
+                });
+                GCHK(hipGetLastError());
+                transpose(tmpt_all, tmp_all, kn, big, n, s);
+            } else {
+                dispatch<T>(big * C, [&](auto nt, auto mt) {
+                    constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
+                    const std::size_t lds = sizeof(cx<T>) * big * C;
+                    lds_attr(k_ap_cols<T, NT, MT>, lds);
+                    k_ap_cols<T, NT, MT><<<dim3((big + C - 1) / C, kn), NT, lds, s>>>(tmp_all, n, d_spec + k0, C, d,
+                                                                                      tw);
+                });
+                GCHK(hipGetLastError());
+            }
             dispatch<T>(big, [&](auto nt, auto mt) {
                 constexpr int NT = decltype(nt)::value, MT = decltype(mt)::value;
                 const std::size_t lds = sizeof(cx<T>) * big;

@@ -773,6 +773,31 @@ int32_t gram_apply_abi(RangGram* h, const T* x, T* out, void* stream) {
     }
 }
 
+bool same_side(const PreparedBase* p, const GramBase* g) {
+    return p->image_nx() == p->image_ny() && p->image_nx() == std::size_t(g->side());
+}
+
+template <class T>
+int32_t defect_abi(RangPrepared* h, RangGram* m, const T* dirty, double q0, const T* factors, int32_t sgn,
+                   const RangFistaSettings* inner, const RangOuterSchedule* schedule, T* x, T* gx,
+                   RangOuterRecord* records, size_t nrecords, RangDefectOutcome* out, void* stream) {
+    return prepared_call<T>(h, [&](PreparedBase* p, std::string& err) {
+        GramBase* g = reinterpret_cast<GramBase*>(m);
+        if (!dirty || !inner || !schedule || !x || !out || !g) return err = "null array", 1;
+        if (g->single() != (sizeof(T) == 4)) return err = "Gram operator of the other precision", 1;
+        if (!same_side(p, g)) return err = "the Gram operator's image side differs from the plan's", 1;
+        if (!(inner->step > 0.0)) return err = "the step must be positive", 1;
+        if (!(schedule->lambda >= 0.0)) return err = "the regularisation weight must be non-negative", 1;
+        if (!(schedule->gamma >= 0.0)) return err = "gamma must be non-negative", 1;
+        std::function<void(const T*, T*, hipStream_t)> cheap = [g](const T* in, T* o, hipStream_t s) {
+            std::string e;
+            if (gram_apply_dev<T>(g, in, o, s, e)) throw std::runtime_error(e);
+        };
+        return prepared_defect<T>(p, cheap, dirty, q0, factors, sgn != 0, *inner, *schedule, x, gx, records,
+                                  nrecords, out, static_cast<hipStream_t>(stream), err);
+    });
+}
+
 template <class T>
 int32_t pcg_abi(RangPrepared* h, RangGram* m, const T* rhs, T* x, int32_t iterations, double tolerance,
                 int32_t* used, double* rel, void* stream) {
@@ -780,6 +805,7 @@ int32_t pcg_abi(RangPrepared* h, RangGram* m, const T* rhs, T* x, int32_t iterat
         GramBase* g = reinterpret_cast<GramBase*>(m);
         if (!rhs || !x || !used || !rel || !g) return err = "null array", 1;
         if (g->single() != (sizeof(T) == 4)) return err = "Gram operator of the other precision", 1;
+        if (!same_side(p, g)) return err = "the Gram operator's image side differs from the plan's", 1;
         int u = 0;
         std::function<void(const T*, T*, hipStream_t)> pre = [g](const T* r, T* z, hipStream_t s) {
             std::string e;
@@ -817,6 +843,13 @@ extern "C" {
                                                  int32_t iterations, double tolerance, int32_t* used,  \
                                                  double* rel, void* stream) {                          \
         return pcg_abi<T>(p, m, rhs, x, iterations, tolerance, used, rel, stream);                      \
+    }                                                                                                   \
+    int32_t rang_amd_wgridder_prepared_defect_##SUF(                                                   \
+        RangPrepared* p, RangGram* cheap, const T* dirty, double q0, const T* factors, int32_t sgn,     \
+        const RangFistaSettings* inner, const RangOuterSchedule* schedule, T* x, T* gx,                 \
+        RangOuterRecord* records, size_t nrecords, RangDefectOutcome* out, void* stream) {              \
+        return defect_abi<T>(p, cheap, dirty, q0, factors, sgn, inner, schedule, x, gx, records,       \
+                             nrecords, out, stream);                                                    \
     }
 RANG_GRAM_ABI(f64, double)
 RANG_GRAM_ABI(f32, float)

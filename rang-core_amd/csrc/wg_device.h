@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "../../include/rang_core_wgridder.h"
@@ -90,6 +91,8 @@ struct PreparedBase {
     virtual std::size_t active() const = 0;
     virtual std::size_t memory() const = 0;
     virtual void release_grids() = 0;
+    virtual std::size_t image_nx() const = 0;
+    virtual std::size_t image_ny() const = 0;
 };
 
 // Device pointers in `in` (uvw, weights f32/f64 or none); freq on the host.
@@ -124,6 +127,11 @@ template <class T>
 int prepared_fista(PreparedBase* p, const T* dirty, double q0, double lam, const T* factors, int sgn,
                    const RangFistaSettings& settings, T* x, T* gx, RangFistaRecord* records,
                    std::size_t nrecords, RangFistaOutcome* out, hipStream_t stream, std::string& err);
+template <class T>
+int prepared_defect(PreparedBase* p, const std::function<void(const T*, T*, hipStream_t)>& cheap,
+                    const T* dirty, double q0, const T* factors, int sgn, const RangFistaSettings& inner,
+                    const RangOuterSchedule& schedule, T* x, T* gx, RangOuterRecord* records,
+                    std::size_t nrecords, RangDefectOutcome* out, hipStream_t stream, std::string& err);
 template <class T>
 int prepared_power(PreparedBase* p, std::uint64_t seed, double tolerance, std::uint64_t max_iterations,
                    double* value, std::uint64_t* used, hipStream_t stream, std::string& err);

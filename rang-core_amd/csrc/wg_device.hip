@@ -166,6 +166,7 @@ enum Slot : int {
     kSlotItemOff2,
     kSlotOcc,  // per (plane group, tile): whether any gridding item reaches it
     kSlotRowNeed,  // per grid row: whether a degridding item reads it
+    kSlotDefect,  // the image vectors of the major cycles (x, G x, G~ x, corrected data)
     kSlotCount
 };
 
@@ -1997,54 +1998,6 @@ __global__ __launch_bounds__(256) void k_correct_cm(const T* __restrict__ dirty,
     }
 }
 
-// Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
-// LDS tiles (64 x 64 in single precision, 32 x 32 in double, a column of
-// padding against bank conflicts): both sides coalesced.
-template <class T>
-__global__ __launch_bounds__(256) void k_transpose(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int R,
-                                                   int C) {
-    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
-    __shared__ cx<T> t[TS][TS + 1];
-    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
-    const long long base = (long long)blockIdx.z * R * C;
-    for (int k = threadIdx.x; k < TS * TS; k += 256) {
-        const int rr = k / TS, cc = k - rr * TS, r = r0 + rr, c = c0 + cc;
-        if (r < R && c < C) t[rr][cc] = in[base + (long long)r * C + c];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < TS * TS; k += 256) {
-        const int cc = k / TS, rr = k - cc * TS, r = r0 + rr, c = c0 + cc;
-        if (r < R && c < C) out[base + (long long)c * R + r] = t[rr][cc];
-    }
-}
-
-// The same for single precision with even R and C: each lane moves two
-// neighbouring values (16 bytes) per access on both sides.
-__global__ __launch_bounds__(256) void k_transpose_pairs(const cx<float>* __restrict__ in, cx<float>* __restrict__ out,
-                                                         int R, int C) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    constexpr int TS = 64;
-    __shared__ cx<float> t[TS][TS + 1];
-    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
-    const long long base = (long long)blockIdx.z * R * C;
-    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
-        const int rr = k / (TS / 2), cp = 2 * (k - rr * (TS / 2)), r = r0 + rr, c = c0 + cp;
-        if (r < R && c < C) {
-            const f4 v = *reinterpret_cast<const f4*>(in + base + (long long)r * C + c);
-            t[rr][cp] = cx<float>{v.x, v.y};
-            t[rr][cp + 1] = cx<float>{v.z, v.w};
-        }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
-        const int cc = k / (TS / 2), rp = 2 * (k - cc * (TS / 2)), r = r0 + rp, c = c0 + cc;
-        if (r < R && c < C) {
-            const cx<float> a = t[rp][cc], b = t[rp + 1][cc];
-            *reinterpret_cast<f4*>(out + base + (long long)c * R + r) = f4{a.re, a.im, b.re, b.im};
-        }
-    }
-}
-
 template <class T>
 void transpose_planes(const cx<T>* in, cx<T>* out, int P, int R, int C, hipStream_t st) {
     constexpr int TS = sizeof(T) == 4 ? 64 : 32;
@@ -2052,12 +2005,12 @@ void transpose_planes(const cx<T>* in, cx<T>* out, int P, int R, int C, hipStrea
     const dim3 grid(unsigned((C + TS - 1) / TS), unsigned((R + TS - 1) / TS), unsigned(P));
     if constexpr (sizeof(T) == 4) {
         if (R % 2 == 0 && C % 2 == 0) {
-            k_transpose_pairs<<<grid, 256, 0, st>>>(in, out, R, C);
+            k_tile_transpose_pairs<float><<<grid, 256, 0, st>>>(in, out, R, C);
             HIPCHK(hipGetLastError());
             return;
         }
     }
-    k_transpose<T><<<grid, 256, 0, st>>>(in, out, R, C);
+    k_tile_transpose<T><<<grid, 256, 0, st>>>(in, out, R, C);
     HIPCHK(hipGetLastError());
 }
 
@@ -4403,6 +4356,8 @@ struct PreparedImpl final : PreparedBase {
     }
     bool single() const override { return sizeof(T) == 4; }
     std::size_t active() const override { return std::size_t(nvis); }
+    std::size_t image_nx() const override { return std::size_t(o.nx); }
+    std::size_t image_ny() const override { return std::size_t(o.ny); }
     std::size_t memory() const override {
         std::size_t b = sizeof(double) * 3 * nrow + wbytes + sizeof(std::uint32_t) * std::size_t(nvis);
         for (int k = 0; k < kSlotCount; ++k) b += ws.size[k];
@@ -4976,6 +4931,158 @@ void fista_partials(Workspace& ws, const double* part, int nblk, hipStream_t st,
     }
 }
 
+// The certificate of x (prox_solver.rs:236-281) from k_fista_eval's sums:
+// misfit, penalty, duality gap and optimality residual.
+struct Certificate {
+    double misfit, pen, gap, kkt;
+};
+Certificate certificate_of(const double (&v)[kFistaSlots], double q0) {
+    const double d_x = v[0];
+    const double misfit = q0 - d_x + 0.5 * v[1];
+    const double pen = v[2], kkt = v[kFistaSums], scale_max = v[kFistaSums + 1];
+    // the dual point s e, e = y - Phi x: <y, e> = 2 Q0 - <d, x>, |e|^2 = 2 Q
+    const double y_e = 2.0 * q0 - d_x, e_e = 2.0 * misfit;
+    const double sc = std::min(std::max(y_e / std::max(e_e, 1e-300), 0.0), scale_max);
+    const double dual = sc * y_e - 0.5 * sc * sc * e_e;
+    return Certificate{misfit, pen, misfit + pen - dual, kkt};
+}
+
+// The FISTA / forward-backward loop of prox_solver.rs:283-445 on any normal
+// operator `apply(in, out)` (out = G in, device vectors of m values on `st`),
+// from x_io, which receives the last iterate (gx_out, when given, its G x).
+// Nine image vectors in `work`; per-block partial sums in `part`.
+template <class T, class Apply>
+void fista_core(Workspace& ws, Apply&& apply, T* work, double* part, long long m, const T* dirty, double q0,
+                double lam, const T* factors, int sgn, const RangFistaSettings& set, T* x_io, T* gx_out,
+                RangFistaRecord* records, std::size_t nrecords, RangFistaOutcome& out, hipStream_t st) {
+    T* x = work;
+    T* x_prev = work + m;
+    T* x_next = work + 2 * m;
+    T* y = work + 3 * m;
+    T* y_prev = work + 4 * m;
+    T* gy = work + 5 * m;
+    T* gy_prev = work + 6 * m;
+    T* gx = work + 7 * m;
+    T* gx_prev = work + 8 * m;
+    HIPCHK(hipMemcpyAsync(x, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(x_prev, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(y, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+    fill_zero(y_prev, sizeof(T) * m, st);
+    fill_zero(gy_prev, sizeof(T) * m, st);
+    fill_zero(gx_prev, sizeof(T) * m, st);
+    const int nblk = red_blocks(m);
+    out = RangFistaOutcome{};
+    out.min_curvature = INFINITY;
+    out.max_curvature = -INFINITY;
+    double t = 1.0, beta = 0.0, last_objective = INFINITY;
+    bool restarted = false;
+    const bool have_gap = !std::isnan(set.gap_tolerance), have_kkt = !std::isnan(set.kkt_tolerance);
+    const bool have_step = !std::isnan(set.step_tolerance);
+    for (std::uint64_t k = 0;; ++k) {
+        apply(static_cast<const T*>(y), gy);
+        k_fista_eval<T><<<nblk, 256, 0, st>>>(y, y_prev, gy, gy_prev, gx_prev, gx, x, x_prev, dirty, factors,
+                                               lam, sgn, beta, m, part);
+        HIPCHK(hipGetLastError());
+        double v[kFistaSlots];
+        fista_partials<T>(ws, part, nblk, st, v);
+        double curvature = NAN;
+        if (k > 0 && v[8] > 0.0) curvature = v[7] / v[8];
+        if (std::isfinite(curvature)) {
+            out.min_curvature = std::fmin(out.min_curvature, curvature);
+            out.max_curvature = std::fmax(out.max_curvature, curvature);
+            if (curvature < 0.0) ++out.negative_curvatures;
+        }
+        const Certificate c = certificate_of(v, q0);
+        RangFistaRecord rec{};
+        rec.iteration = k;
+        rec.objective = c.misfit + c.pen;
+        rec.misfit = c.misfit;
+        rec.penalty = c.pen;
+        rec.gap = c.gap;
+        rec.kkt = c.kkt;
+        rec.nonzero = std::uint64_t(v[6]);
+        rec.flux = v[5];
+        rec.curvature = curvature;
+        rec.relative_step = std::sqrt(v[3]) / std::max(std::sqrt(v[4]), 1e-300);
+        rec.restarted = restarted ? 1 : 0;
+        if (records && k < nrecords) records[k] = rec;
+        if (std::isfinite(rec.objective)) {
+            if (rec.objective > last_objective) {
+                ++out.objective_rises;
+                out.largest_rise = std::fmax(out.largest_rise, rec.objective - last_objective);
+            }
+            last_objective = rec.objective;
+        }
+        out.last = rec;
+        out.iterations = k;
+        const bool converged = (have_gap && c.gap <= set.gap_tolerance) || (have_kkt && c.kkt <= set.kkt_tolerance) ||
+                               (k > 0 && have_step && rec.relative_step <= set.step_tolerance);
+        if (converged || k == set.max_iterations) {
+            out.converged = converged ? 1 : 0;
+            HIPCHK(hipMemcpyAsync(x_io, x, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+            if (gx_out) HIPCHK(hipMemcpyAsync(gx_out, gx, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            return;
+        }
+        k_fista_step<T><<<nblk, 256, 0, st>>>(y, dirty, gy, x, x_next, factors, lam, sgn, set.step, m, part);
+        HIPCHK(hipGetLastError());
+        const double inner = sum_partials<T>(ws, part, nblk, st);
+        double next_beta = 0.0;
+        restarted = false;
+        if (set.momentum) {
+            if (set.restart && inner > 0.0) {
+                t = 1.0;
+                restarted = true;
+                ++out.restarts;
+            } else {
+                const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+                next_beta = (t - 1.0) / t_next;
+                t = t_next;
+            }
+        }
+        std::swap(y, y_prev);
+        std::swap(gy, gy_prev);
+        std::swap(gx, gx_prev);
+        std::swap(x_prev, x);
+        std::swap(x, x_next);
+        k_fista_extrap<T><<<nblk, 256, 0, st>>>(y, x, x_prev, next_beta, m);
+        HIPCHK(hipGetLastError());
+        beta = next_beta;
+    }
+}
+
+// The defect of a cheap operator at x (prox_solver.rs:797-812, 838-850):
+// corrected = d - (G x - G~ x); per block the largest |G x - G~ x| and the
+// largest residual d - G x (|d - G x| when signed), both floored at 0.
+template <class T>
+__global__ __launch_bounds__(256) void k_defect(const T* d, const T* g, const T* gc, T* corrected, int sgn,
+                                                long long n, double* part) {
+    double sup = 0.0, peak = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double gi = double(g[i]), di = double(d[i]);
+        const double c = gi - double(gc[i]);
+        corrected[i] = T(di - c);
+        sup = fmax(sup, fabs(c));
+        const double r = di - gi;
+        peak = fmax(peak, sgn ? fabs(r) : r);
+    }
+    __shared__ double s[2][256];
+    s[0][threadIdx.x] = sup;
+    s[1][threadIdx.x] = peak;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            s[0][threadIdx.x] = fmax(s[0][threadIdx.x], s[0][threadIdx.x + h]);
+            s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (long long)blockIdx.x] = s[0][0];
+        part[2 * (long long)blockIdx.x + 1] = s[1][0];
+    }
+}
+
 }  // namespace
 
 // FISTA or forward-backward on the prepared plan's normal operator
@@ -4996,107 +5103,103 @@ int prepared_fista(PreparedBase* base, const T* dirty, double q0, double lam, co
         const long long m = (long long)p.o.nx * p.o.ny;
         const std::size_t vb = sizeof(T) * std::max<long long>(m, 1);
         T* work = static_cast<T*>(p.ws.get(kSlotFista, vb * 9));
-        T* x = work;
-        T* x_prev = work + m;
-        T* x_next = work + 2 * m;
-        T* y = work + 3 * m;
-        T* y_prev = work + 4 * m;
-        T* gy = work + 5 * m;
-        T* gy_prev = work + 6 * m;
-        T* gx = work + 7 * m;
-        T* gx_prev = work + 8 * m;
-        HIPCHK(hipMemcpyAsync(x, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(x_prev, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(y, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
-        fill_zero(y_prev, sizeof(T) * m, st);
-        fill_zero(gy_prev, sizeof(T) * m, st);
-        fill_zero(gx_prev, sizeof(T) * m, st);
-        const int nblk = red_blocks(m);
         double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kFistaSlots * kRedBlocks));
         RangFistaOutcome out{};
-        out.min_curvature = INFINITY;
-        out.max_curvature = -INFINITY;
-        double t = 1.0, beta = 0.0, last_objective = INFINITY;
-        bool restarted = false;
-        const bool have_gap = !std::isnan(set.gap_tolerance), have_kkt = !std::isnan(set.kkt_tolerance);
-        const bool have_step = !std::isnan(set.step_tolerance);
-        for (std::uint64_t k = 0;; ++k) {
-            gram_apply<T>(p, y, gy, st, keep, timer);
-            k_fista_eval<T><<<nblk, 256, 0, st>>>(y, y_prev, gy, gy_prev, gx_prev, gx, x, x_prev, dirty,
-                                                   factors, lam, sgn, beta, m, part);
+        fista_core<T>(p.ws, [&](const T* in, T* o) { gram_apply<T>(p, in, o, st, keep, timer); }, work, part, m,
+                      dirty, q0, lam, factors, sgn, set, x_io, gx_out, records, nrecords, out, st);
+        timer.finish();
+        *outc = out;
+        return 0;
+    } catch (const DevError& e) {
+        err = e.msg;
+        return 2;
+    }
+}
+
+// Major cycles on a cheap normal operator with its defect moved into the data
+// (core/src/prox_solver.rs:760-875, defect_correction): each cycle applies the
+// exact operator (the prepared plan: vis2dirty of dirty2vis) and the cheap one
+// at x_j, judges x_j by the exact certificate at the target lambda, and solves
+// min 1/2 <x, G~ x> - <d - (G - G~) x_j, x> + R(x) by FISTA on the cheap
+// operator from x_j, at max(lambda, gamma * peak residual) and without a gap
+// test. At a fixed point the cheap operator's error cancels. One record per
+// cycle; x_io receives the last judged iterate and gx_out its exact G x.
+template <class T>
+int prepared_defect(PreparedBase* base, const std::function<void(const T*, T*, hipStream_t)>& cheap,
+                    const T* dirty, double q0, const T* factors, int sgn, const RangFistaSettings& inner,
+                    const RangOuterSchedule& sched, T* x_io, T* gx_out, RangOuterRecord* records,
+                    std::size_t nrecords, RangDefectOutcome* outc, hipStream_t st, std::string& err) {
+    auto& p = *static_cast<PreparedImpl<T>*>(base);
+    try {
+        std::lock_guard<std::mutex> lk(p.ws.mu);
+        HostKeep keep;
+        StageTimer timer(st);
+        const long long m = (long long)p.o.nx * p.o.ny;
+        const std::size_t vb = sizeof(T) * std::max<long long>(m, 1);
+        T* work = static_cast<T*>(p.ws.get(kSlotFista, vb * 9));
+        T* outer = static_cast<T*>(p.ws.get(kSlotDefect, vb * 4));
+        T *x = outer, *g = outer + m, *gc = outer + 2 * m, *corrected = outer + 3 * m;
+        double* part = static_cast<double*>(p.ws.get(kSlotRed, sizeof(double) * kFistaSlots * kRedBlocks));
+        const int nblk = red_blocks(m);
+        HIPCHK(hipMemcpyAsync(x, x_io, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+        RangFistaSettings set = inner;
+        set.gap_tolerance = NAN;  // the inner objective is not of least-squares form
+        RangDefectOutcome out{};
+        std::uint64_t inner_iterations = 0;
+        for (std::uint64_t j = 0;; ++j) {
+            gram_apply<T>(p, x, g, st, keep, timer);
+            ++out.exact_applies;
+            // the certificate at the target lambda: k_fista_eval with y = x
+            // and beta = 0 (its G x written over g, unchanged)
+            k_fista_eval<T><<<nblk, 256, 0, st>>>(x, x, g, g, g, g, x, x, dirty, factors, sched.lambda, sgn, 0.0,
+                                                   m, part);
             HIPCHK(hipGetLastError());
             double v[kFistaSlots];
             fista_partials<T>(p.ws, part, nblk, st, v);
-            double curvature = NAN;
-            if (k > 0 && v[8] > 0.0) curvature = v[7] / v[8];
-            if (std::isfinite(curvature)) {
-                out.min_curvature = std::fmin(out.min_curvature, curvature);
-                out.max_curvature = std::fmax(out.max_curvature, curvature);
-                if (curvature < 0.0) ++out.negative_curvatures;
+            const Certificate c = certificate_of(v, q0);
+            cheap(x, gc, st);
+            ++out.cheap_applies;
+            k_defect<T><<<nblk, 256, 0, st>>>(dirty, g, gc, corrected, sgn, m, part);
+            HIPCHK(hipGetLastError());
+            double* h = static_cast<double*>(p.ws.host(sizeof(double) * 2 * nblk));
+            HIPCHK(hipMemcpyAsync(h, part, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            double sup = 0.0, peak = 0.0;
+            for (int b = 0; b < nblk; ++b) {
+                sup = std::fmax(sup, h[2 * b]);
+                peak = std::fmax(peak, h[2 * b + 1]);
             }
-            // certificate (prox_solver.rs:236-281)
-            const double d_x = v[0];
-            const double misfit = q0 - d_x + 0.5 * v[1];
-            const double pen = v[2], kkt = v[kFistaSums], scale_max = v[kFistaSums + 1];
-            const double y_e = 2.0 * q0 - d_x, e_e = 2.0 * misfit;
-            const double sc = std::min(std::max(y_e / std::max(e_e, 1e-300), 0.0), scale_max);
-            const double dual = sc * y_e - 0.5 * sc * sc * e_e;
-            const double gap = misfit + pen - dual;
-            RangFistaRecord rec{};
-            rec.iteration = k;
-            rec.objective = misfit + pen;
-            rec.misfit = misfit;
-            rec.penalty = pen;
-            rec.gap = gap;
-            rec.kkt = kkt;
-            rec.nonzero = std::uint64_t(v[6]);
-            rec.flux = v[5];
-            rec.curvature = curvature;
-            rec.relative_step = std::sqrt(v[3]) / std::max(std::sqrt(v[4]), 1e-300);
-            rec.restarted = restarted ? 1 : 0;
-            if (records && k < nrecords) records[k] = rec;
-            if (std::isfinite(rec.objective)) {
-                if (rec.objective > last_objective) {
-                    ++out.objective_rises;
-                    out.largest_rise = std::fmax(out.largest_rise, rec.objective - last_objective);
-                }
-                last_objective = rec.objective;
-            }
+            RangOuterRecord rec{};
+            rec.outer = j;
+            rec.objective = c.misfit + c.pen;
+            rec.misfit = c.misfit;
+            rec.gap = c.gap;
+            rec.kkt = c.kkt;
+            rec.inner_iterations = inner_iterations;
+            rec.defect_sup = sup;
+            if (records && j < nrecords) records[j] = rec;
             out.last = rec;
-            out.iterations = k;
-            const bool converged = (have_gap && gap <= set.gap_tolerance) || (have_kkt && kkt <= set.kkt_tolerance) ||
-                                   (k > 0 && have_step && rec.relative_step <= set.step_tolerance);
-            if (converged || k == set.max_iterations) {
-                out.converged = converged ? 1 : 0;
+            out.cycles = j;
+            if (c.kkt <= sched.kkt || j == sched.cycles) {
+                out.converged = c.kkt <= sched.kkt ? 1 : 0;
+                out.iterations = inner_iterations;
                 HIPCHK(hipMemcpyAsync(x_io, x, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
-                if (gx_out) HIPCHK(hipMemcpyAsync(gx_out, gx, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
+                if (gx_out) HIPCHK(hipMemcpyAsync(gx_out, g, sizeof(T) * m, hipMemcpyDeviceToDevice, st));
                 HIPCHK(hipStreamSynchronize(st));
                 break;
             }
-            k_fista_step<T><<<nblk, 256, 0, st>>>(y, dirty, gy, x, x_next, factors, lam, sgn, set.step, m, part);
-            HIPCHK(hipGetLastError());
-            const double inner = sum_partials<T>(p.ws, part, nblk, st);
-            double next_beta = 0.0;
-            restarted = false;
-            if (set.momentum) {
-                if (set.restart && inner > 0.0) {
-                    t = 1.0;
-                    restarted = true;
-                    ++out.restarts;
-                } else {
-                    const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
-                    next_beta = (t - 1.0) / t_next;
-                    t = t_next;
-                }
-            }
-            std::swap(y, y_prev);
-            std::swap(gy, gy_prev);
-            std::swap(gx, gx_prev);
-            std::swap(x_prev, x);
-            std::swap(x, x_next);
-            k_fista_extrap<T><<<nblk, 256, 0, st>>>(y, x, x_prev, next_beta, m);
-            HIPCHK(hipGetLastError());
-            beta = next_beta;
+            const double lam = std::fmax(sched.lambda, sched.gamma * peak);
+            RangFistaOutcome io{};
+            fista_core<T>(
+                p.ws,
+                [&](const T* in, T* o) {
+                    cheap(in, o, st);
+                    ++out.cheap_applies;
+                },
+                work, part, m, corrected, q0, lam, factors, sgn, set, x, nullptr, nullptr, 0, io, st);
+            inner_iterations += io.iterations;
+            out.restarts += io.restarts;
+            out.negative_curvatures += io.negative_curvatures;
         }
         timer.finish();
         *outc = out;
@@ -5106,6 +5209,15 @@ int prepared_fista(PreparedBase* base, const T* dirty, double q0, double lam, co
         return 2;
     }
 }
+
+template int prepared_defect<float>(PreparedBase*, const std::function<void(const float*, float*, hipStream_t)>&,
+                                    const float*, double, const float*, int, const RangFistaSettings&,
+                                    const RangOuterSchedule&, float*, float*, RangOuterRecord*, std::size_t,
+                                    RangDefectOutcome*, hipStream_t, std::string&);
+template int prepared_defect<double>(PreparedBase*, const std::function<void(const double*, double*, hipStream_t)>&,
+                                     const double*, double, const double*, int, const RangFistaSettings&,
+                                     const RangOuterSchedule&, double*, double*, RangOuterRecord*, std::size_t,
+                                     RangDefectOutcome*, hipStream_t, std::string&);
 
 // The largest eigenvalue of the normal operator by power iteration from the
 // reference's seeded start (core/src/prox_solver.rs:886-908): the estimate

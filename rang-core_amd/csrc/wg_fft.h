@@ -269,4 +269,54 @@ __device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw,
     }
 }
 
+// Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
+// LDS tiles (64 x 64 in single precision, 32 x 32 in double, a column of
+// padding against bank conflicts): both sides coalesced.
+template <class T>
+__global__ __launch_bounds__(256) void k_tile_transpose(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int R,
+                                                        int C) {
+    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+    __shared__ cx<T> t[TS][TS + 1];
+    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
+    const long long base = (long long)blockIdx.z * R * C;
+    for (int k = threadIdx.x; k < TS * TS; k += 256) {
+        const int rr = k / TS, cc = k - rr * TS, r = r0 + rr, c = c0 + cc;
+        if (r < R && c < C) t[rr][cc] = in[base + (long long)r * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TS * TS; k += 256) {
+        const int cc = k / TS, rr = k - cc * TS, r = r0 + rr, c = c0 + cc;
+        if (r < R && c < C) out[base + (long long)c * R + r] = t[rr][cc];
+    }
+}
+
+// The same for single precision with even R and C: each lane moves two
+// neighbouring values (16 bytes) per access on both sides.
+template <class T>
+__global__ __launch_bounds__(256) void k_tile_transpose_pairs(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                                                              int R, int C) {
+    static_assert(sizeof(T) == 4, "pairs of single-precision values");
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int TS = 64;
+    __shared__ cx<T> t[TS][TS + 1];
+    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
+    const long long base = (long long)blockIdx.z * R * C;
+    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
+        const int rr = k / (TS / 2), cp = 2 * (k - rr * (TS / 2)), r = r0 + rr, c = c0 + cp;
+        if (r < R && c < C) {
+            const f4 v = *reinterpret_cast<const f4*>(in + base + (long long)r * C + c);
+            t[rr][cp] = cx<T>{v.x, v.y};
+            t[rr][cp + 1] = cx<T>{v.z, v.w};
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TS * TS / 2; k += 256) {
+        const int cc = k / (TS / 2), rp = 2 * (k - cc * (TS / 2)), r = r0 + rp, c = c0 + cc;
+        if (r < R && c < C) {
+            const cx<T> a = t[rp][cc], b = t[rp + 1][cc];
+            *reinterpret_cast<f4*>(out + base + (long long)c * R + r) = f4{a.re, a.im, b.re, b.im};
+        }
+    }
+}
+
 }  // namespace rang

@@ -135,6 +135,47 @@ class FistaOutcome(ctypes.Structure):
     ]
 
 
+class OuterSchedule(ctypes.Structure):
+    """RangOuterSchedule: prox_solver::OuterSchedule (core/src/prox_solver.rs:760-772)."""
+
+    _fields_ = [
+        ("cycles", ctypes.c_uint64),
+        ("kkt", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+    ]
+
+
+class OuterRecord(ctypes.Structure):
+    """RangOuterRecord: prox_solver::OuterRecord (core/src/prox_solver.rs:447-462)."""
+
+    _fields_ = [
+        ("outer", ctypes.c_uint64),
+        ("objective", ctypes.c_double),
+        ("misfit", ctypes.c_double),
+        ("gap", ctypes.c_double),
+        ("kkt", ctypes.c_double),
+        ("inner_iterations", ctypes.c_uint64),
+        ("defect_sup", ctypes.c_double),
+    ]
+
+
+class DefectOutcome(ctypes.Structure):
+    """RangDefectOutcome: how defect_correction ended (core/src/prox_solver.rs:778-875)."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_uint64),
+        ("converged", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("cycles", ctypes.c_uint64),
+        ("restarts", ctypes.c_uint64),
+        ("negative_curvatures", ctypes.c_uint64),
+        ("exact_applies", ctypes.c_uint64),
+        ("cheap_applies", ctypes.c_uint64),
+        ("last", OuterRecord),
+    ]
+
+
 class GridInputs(ctypes.Structure):
     """`GridInputs` (core/src/ffi_grid.rs:7-29)."""
 
@@ -306,6 +347,10 @@ SIGNATURES = {
     "rang_amd_gram_free": ([_vp], None),
     "rang_amd_wgridder_prepared_pcg_f64": ([_vp, _vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
     "rang_amd_wgridder_prepared_pcg_f32": ([_vp, _vp, _vp, _vp, _i32, _dp, _vp, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_defect_f64": (
+        [_vp, _vp, _vp, _dp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
+    "rang_amd_wgridder_prepared_defect_f32": (
+        [_vp, _vp, _vp, _dp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp], _i32),
     # imaging weights and Stokes I (include/rang_core_weights.h)
     "rang_core_weight_grid_shape": ([_sz, _sz, _dp, _vp], _i32),
     "rang_core_padded_size": ([_sz, _dp], _sz),

@@ -63,6 +63,41 @@ def stokes_i(data, weights, flags):
     return vis, weight
 
 
+def stokes_i_device(data, weights, flags):
+    """stokes_i on device tensors (rang_amd_stokes_i_dev): `data` complex64
+    (..., 4), `weights` float32 and `flags` uint8 or bool of the same shape,
+    all contiguous on one GPU; returns complex64 and float32 tensors of
+    data.shape[:-1] on that GPU, formed on the caller's current stream."""
+    import ctypes
+
+    import torch
+
+    if data.dtype != torch.complex64 or weights.dtype != torch.float32:
+        raise ValueError("complex64 data and float32 weights required")
+    if flags.dtype == torch.bool:
+        flags = flags.view(torch.uint8)
+    if flags.dtype != torch.uint8:
+        raise ValueError("uint8 or bool flags required")
+    if data.shape[-1] != 4 or weights.shape != data.shape or flags.shape != data.shape:
+        raise ValueError("data, weights and flags with four correlations required")
+    if not (data.is_contiguous() and weights.is_contiguous() and flags.is_contiguous()):
+        raise ValueError("data, weights and flags must be contiguous")
+    if not (data.device == weights.device == flags.device) or data.device.type != "cuda":
+        raise ValueError("data, weights and flags must be on one GPU")
+    shape = data.shape[:-1]
+    vis = torch.empty(shape, dtype=torch.complex64, device=data.device)
+    weight = torch.empty(shape, dtype=torch.float32, device=data.device)
+    ncell = vis.numel()
+    if ncell:
+        stream = torch.cuda.current_stream(data.device).cuda_stream
+        inp = _core.StokesIInputs(ncell, data.data_ptr(), weights.data_ptr(), flags.data_ptr())
+        out = _core.StokesIOutputs(vis.data_ptr(), weight.data_ptr())
+        code = _core.library().rang_amd_stokes_i_dev(ctypes.byref(inp), ctypes.byref(out), ctypes.c_void_p(stream))
+        if code:
+            raise RuntimeError(f"core Stokes I failed with code {code}")
+    return vis, weight
+
+
 def imaging_weights(uvw, wavelength, weights, row_selected, mode="briggs", robust=0.5, width=None, height=None,
                     pixel_scale=None, super_weight=1.0, rank_filter=DEFAULT_RANK_FILTER, return_grid=True):
     """WSClean imaging weights for every row and channel: `uvw` (nrow, 3)

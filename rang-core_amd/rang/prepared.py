@@ -148,6 +148,20 @@ class Prepared:
                                 ctypes.byref(used), ctypes.byref(rel), _stream()))
         return x, used.value, rel.value
 
+    def defect(self, cheap, dirty, q0, x, *, factors=None, signed=False, inner, schedule, gx=None, nrecords=0):
+        """prox_solver::defect_correction (core/src/prox_solver.rs:778-875) with
+        the L1 prior: major cycles on this plan's normal operator, each inner
+        FISTA solve on `cheap` (a rang.gram.GramOperator of the same side and
+        precision). x (the start) is overwritten with the result; returns
+        (outcome, records[:min(cycles + 1, nrecords)])."""
+        recs = (_core.OuterRecord * max(int(nrecords), 1))()
+        out = _core.DefectOutcome()
+        _check(self._fn("defect")(self._h, cheap._h, _ptr(dirty), float(q0), _ptr(factors), int(bool(signed)),
+                                   ctypes.byref(inner), ctypes.byref(schedule), _ptr(x), _ptr(gx), recs,
+                                   int(nrecords), ctypes.byref(out), _stream()))
+        n = min(int(out.cycles) + 1, int(nrecords))
+        return out, [recs[k] for k in range(n)]
+
     def sorted(self, dense):
         """Prepared::sorted (core/src/wgridder.rs:3269-3286): `dense` (nrow x
         nchan, 4-, 8- or 16-byte elements) at the active visibilities, in the

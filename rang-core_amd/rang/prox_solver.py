@@ -1,6 +1,8 @@
 """Forward-backward and FISTA for 1/2 |y - Phi m|^2_W + R(m) on the GPU, as
 rang-core's `prox_solver` (core/src/prox_solver.rs): `solve` (:283-445) with
-the `L1Prior` (:55-132) over a `WgridMeasurement`, and `power_iteration`
+the `L1Prior` (:55-132) over a `WgridMeasurement`, `defect_correction`
+(:760-875: major cycles judged by the exact operator, each solved on a cheap
+one such as rang.gram's visibility-free operator), and `power_iteration`
 (:886-908) for the step 1 / L.
 
 The iteration runs natively on the device (one Gram application per
@@ -146,6 +148,86 @@ def solve(measurement, problem, prior, settings, x0, *, keep_records=True):
                    objective_rises=int(out.objective_rises), largest_rise=out.largest_rise,
                    negative_curvatures=int(out.negative_curvatures), min_curvature=out.min_curvature,
                    max_curvature=out.max_curvature, records=[_record(r) for r in recs])
+
+
+@dataclass
+class OuterSchedule:
+    """prox_solver.rs:760-772: at most `cycles` cycles; stop when the exact
+    optimality residual at the target `lam` is <= kkt; a cycle solves at
+    max(lam, gamma * peak residual) (0 keeps lam)."""
+
+    cycles: int
+    kkt: float
+    lam: float
+    gamma: float = 0.0
+
+
+@dataclass
+class OuterRecord:
+    """One major cycle (prox_solver.rs:447-462)."""
+
+    outer: int
+    objective: float
+    misfit: float
+    gap: float
+    kkt: float
+    inner_iterations: int
+    defect_sup: float
+
+
+@dataclass
+class DefectOutcome:
+    """What defect_correction returns (prox_solver.rs:778-875): the judged
+    iterate, its exact G x, the last cycle's record, the inner iterations
+    summed, the stop ("converged" or "cycles"), and the applications of each
+    operator."""
+
+    x: torch.Tensor
+    gx: torch.Tensor
+    last: OuterRecord
+    iterations: int
+    stop: str
+    restarts: int
+    negative_curvatures: int
+    exact_applies: int
+    cheap_applies: int
+    records: List[OuterRecord] = field(default_factory=list)
+
+
+def _outer(r):
+    return OuterRecord(int(r.outer), r.objective, r.misfit, r.gap, r.kkt, int(r.inner_iterations), r.defect_sup)
+
+
+def defect_correction(exact, cheap, problem, prior, inner, schedule, x0):
+    """Solve on a cheap operator with its error at x_j moved into the data:
+    min 1/2 <x, G~ x> - <d - (G - G~) x_j, x> + R(x), cycle after cycle, each
+    cycle judged by the exact operator's certificate at schedule.lam
+    (prox_solver.rs:778-875). `exact` is a WgridMeasurement, `cheap` a
+    rang.gram.GramOperator of the same side and precision (for instance
+    GramOperator.new of the same uvw and normalised weights); `inner` the
+    FISTA settings of each inner solve (its gap tolerance is not used). The
+    whole loop runs natively on the device."""
+    if not isinstance(prior, L1Prior):
+        raise TypeError("the device solver takes the L1 prior")
+    plan = exact.plan
+    real = torch.float32 if plan.single else torch.float64
+    if cheap.single != plan.single:
+        raise ValueError("the cheap operator's precision differs from the measurement's")
+    x = x0.to(real).contiguous().clone()
+    gx = torch.empty_like(x)
+    dirty = problem.dirty.to(real).contiguous()
+    factors = None if prior.factors is None else prior.factors.to(real).contiguous()
+    s = _core.FistaSettings(float(inner.step), int(inner.max_iterations), int(bool(inner.momentum)),
+                            int(bool(inner.restart)), math.nan, _tol(inner.kkt_tolerance),
+                            _tol(inner.step_tolerance))
+    sched = _core.OuterSchedule(int(schedule.cycles), float(schedule.kkt), float(schedule.lam),
+                                float(schedule.gamma))
+    out, recs = plan.defect(cheap, dirty, problem.q0, x, factors=factors, signed=prior.signed, inner=s,
+                            schedule=sched, gx=gx, nrecords=int(schedule.cycles) + 1)
+    return DefectOutcome(x=x, gx=gx, last=_outer(out.last), iterations=int(out.iterations),
+                         stop="converged" if out.converged else "cycles", restarts=int(out.restarts),
+                         negative_curvatures=int(out.negative_curvatures), exact_applies=int(out.exact_applies),
+                         cheap_applies=int(out.cheap_applies), records=[_outer(r) for r in recs])
 
 
 def power_iteration(measurement, seed, tolerance, max_iterations):

@@ -156,6 +156,56 @@ def config2_device(device, seed=1234, ntime=900, nchan=256):
     return case
 
 
+def fullpol_case_device(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, device, *, nant=64, radius_m=8000.0,
+                        lat_deg=-30.7, dec_deg=-45.0, dt_s=8.0, nsrc=20, sky_radius_deg=1.4,
+                        flux=(1e-3, 1.0), noise_jy=1e-3, seed=1234, rows_per_chunk=16384):
+    """Four correlations XX, XY, YX, YY per visibility (complex64, (nrow,
+    nchan, 4)) of an unpolarised point-source sky spread over the field: XX =
+    YY = I + noise, XY = YX = noise (independent complex Gaussian noise of
+    sigma `noise_jy` in each), the direct sum in float64 on the device; the
+    same array and tracks as imaging_case_device. Also unit weights per
+    correlation (float32, (nrow, nchan, 4)) and no flags (uint8). Returns
+    torch tensors on `device`, the sky, and the largest |w| in wavelengths."""
+    import torch
+
+    enu = antennas(nant, radius_m, seed)
+    uvw = uvw_tracks(baselines_xyz(enu, lat_deg), ntime, dt_s, dec_deg)
+    freq = np.linspace(f_lo, f_hi, nchan)
+    flux_s, l_s, m_s = point_sky(nsrc, sky_radius_deg, flux[0], flux[1], seed + 7)
+    d_uvw = torch.from_numpy(uvw).to(device)
+    f = torch.from_numpy(freq / C_LIGHT).to(device)
+    nrow = uvw.shape[0]
+    data = torch.empty((nrow, nchan, 4), dtype=torch.complex64, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed + 1)
+    for r0 in range(0, nrow, rows_per_chunk):
+        r1 = min(nrow, r0 + rows_per_chunk)
+        u = d_uvw[r0:r1]
+        acc = torch.zeros((r1 - r0, nchan), dtype=torch.complex128, device=device)
+        for s in range(flux_s.size):
+            n1 = np.sqrt(1.0 - l_s[s] ** 2 - m_s[s] ** 2) - 1.0
+            path = u[:, 0] * l_s[s] + u[:, 1] * m_s[s] + u[:, 2] * n1
+            ph = (-2.0 * np.pi) * path[:, None] * f[None, :]
+            acc += flux_s[s] * torch.polar(torch.ones_like(ph), ph)
+        noise = noise_jy * torch.randn((r1 - r0, nchan, 4, 2), dtype=torch.float32, device=device, generator=gen)
+        block = torch.complex(noise[..., 0], noise[..., 1])
+        sky = acc.to(torch.complex64)
+        block[:, :, 0] += sky
+        block[:, :, 3] += sky
+        data[r0:r1] = block
+    return {
+        "uvw": d_uvw,
+        "freq": torch.from_numpy(freq).to(device),
+        "data": data,
+        "weight": torch.ones((nrow, nchan, 4), dtype=torch.float32, device=device),
+        "flag": torch.zeros((nrow, nchan, 4), dtype=torch.uint8, device=device),
+        "npix": npix,
+        "pixsize": pix_arcsec * ARCSEC,
+        "sky": (flux_s, l_s, m_s),
+        "wmax_lambda": float(np.abs(uvw[:, 2]).max() * freq.max() / C_LIGHT),
+    }
+
+
 def flag_runs(ntime, nbl, nchan, fraction, rng, max_time=8):
     """(ntime * nbl, nchan) bool flags: rectangles of contiguous time steps
     (1..max_time, all baselines of those steps) by contiguous channels

@@ -150,3 +150,31 @@ def test_nodes_in_several_passes_give_the_same_operator(monkeypatch):
     many = gram.GramOperator.new(opts(n, c["cell"], 1e-10), uvw, freq, wgt, ex, 1e-10, single=False)
     got = many.apply(x).cpu().numpy()
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_transposed_column_pass_gives_the_same_operator(monkeypatch, single):
+    # large images transpose the half-transformed slices around the column
+    # pass (whole columns read and written); forced on a small one, in one and
+    # in several passes, it must give the in-place form's result
+    from rang import gram
+
+    n = 64
+    c = gram_case(n, 800, 2000.0, False)
+    dtype = torch.float32 if single else torch.float64
+    uvw, freq, wgt = dev(c, dtype)
+    x = torch.from_numpy(model_of(n)).cuda().to(dtype)
+    ex = gram.WExpansion(10, 1.0, 1e-13)
+    eps = 1e-5 if single else 1e-10
+    monkeypatch.setenv("RANG_GRAM_TRANSPOSE", "0")
+    plain = gram.GramOperator.new(opts(n, c["cell"], eps), uvw, freq, wgt, ex, eps, single=single)
+    want = plain.apply(x).cpu().numpy()
+    monkeypatch.setenv("RANG_GRAM_TRANSPOSE", "1")
+    for budget in (None, "1"):
+        if budget:
+            monkeypatch.setenv("RANG_GRAM_PASS_BYTES", budget)
+        op = gram.GramOperator.new(opts(n, c["cell"], eps), uvw, freq, wgt, ex, eps, single=single)
+        got = op.apply(x).cpu().numpy()
+        print(f"[gram] transposed column pass, single {single}, budget {budget}: "
+              f"{np.abs(got - want).max() / np.abs(want).max():.2e}", flush=True)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()

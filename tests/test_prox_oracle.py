@@ -141,3 +141,74 @@ def test_certificate_gap_vanishes_at_the_exact_minimiser():
     out = X.solve(lambda v: v, d, q0, prior, np.zeros(50), step=1.0, max_iterations=5, gap_tolerance=1e-12)
     assert out["stop"] == "converged"
     assert np.allclose(out["x"], x, atol=1e-15)
+
+
+def test_defect_correction_reaches_the_exact_problem_from_a_wrong_operator():
+    # prox_solver.rs:1301-1376: the cheap operator is the exact one scaled by
+    # 1.2; the cycles converge to the exact problem's objective, while the
+    # wrong operator alone gives a clearly different solution
+    gram, d, q0 = dense_problem(6, 20)
+    lmax, _ = X.power_iteration(gram, 36, 3, 1e-8, 2000)
+
+    def wrong(x):
+        return 1.2 * gram(x)
+
+    prior = X.L1Prior(0.02, 1.0)
+    seen = []
+    out = X.defect_correction(gram, wrong, d, q0, prior, np.zeros(36), step=1.0 / (1.3 * lmax),
+                              max_iterations=400, kkt_tolerance=1e-7, cycles=60, kkt=1e-6, lam=0.02,
+                              gamma=0.0, observer=lambda r, x: seen.append(r))
+    assert out["stop"] == "converged"
+    direct = X.solve(gram, d, q0, prior, np.zeros(36), step=1.0 / (1.01 * lmax), max_iterations=20000,
+                     gap_tolerance=1e-12)
+    assert abs(out["last"]["objective"] - direct["last"]["objective"]) < 1e-7
+    alone = X.solve(wrong, d, q0, prior, np.zeros(36), step=1.0 / (1.3 * lmax), max_iterations=20000,
+                    kkt_tolerance=1e-7)
+    assert float(np.max(np.abs(alone["x"] - direct["x"]))) > 1e-2
+    assert len(seen) > 1
+    # one exact and one cheap application per cycle, plus the inner solves'
+    assert out["exact_applies"] == len(seen)
+    assert out["records"][-1]["kkt"] <= 1e-6
+    # the defect at the fixed point is the operators' difference at x: 0.2 G x
+    assert math.isclose(out["records"][-1]["defect_sup"], float(np.max(np.abs(0.2 * gram(out["x"])))),
+                        rel_tol=1e-12)
+
+
+def test_defect_correction_with_the_exact_operator_is_one_solve():
+    # G~ = G: the correction vanishes, so the first cycle's inner solve is the
+    # plain solve from x0, and the next certificate judges its iterate
+    gram, d, q0 = dense_problem(6, 20)
+    lmax, _ = X.power_iteration(gram, 36, 3, 1e-8, 2000)
+    kw = dict(step=1.0 / (1.01 * lmax), max_iterations=150)
+    out = X.defect_correction(gram, gram, d, q0, X.L1Prior(0.02, 1.0), np.zeros(36), cycles=1, kkt=0.0,
+                              lam=0.02, **kw)
+    plain = X.solve(gram, d, q0, X.L1Prior(0.02, 1.0), np.zeros(36), **kw)
+    assert out["stop"] == "cycles"
+    assert np.array_equal(out["x"], plain["x"])
+    assert out["records"][0]["defect_sup"] == 0.0
+    assert out["iterations"] == plain["iterations"] == 150
+    m, p, _, k = X.certificate(d, q0, plain["x"], gram(plain["x"]), X.L1Prior(0.02, 1.0))
+    assert math.isclose(out["last"]["objective"], m + p, rel_tol=1e-13)
+    assert math.isclose(out["last"]["kkt"], k, rel_tol=1e-13)
+
+
+def test_defect_correction_with_the_sign_of_the_defect_flipped_misses_the_exact_problem(monkeypatch):
+    # a planted mistake in the oracle: d + (G - G~) x_j instead of d - (...);
+    # the cycles then settle away from the exact problem's objective
+    gram, d, q0 = dense_problem(6, 20)
+    lmax, _ = X.power_iteration(gram, 36, 3, 1e-8, 2000)
+    orig = X.solve
+
+    def flipped(cheap, corrected, q0_, prior, x, **kw):
+        # corrected = d - c arrives here; hand the inner solve d + c
+        return orig(cheap, 2.0 * d - corrected, q0_, prior, x, **kw)
+
+    monkeypatch.setattr(X, "solve", flipped)
+    out = X.defect_correction(gram, lambda x: 1.2 * gram(x), d, q0, X.L1Prior(0.02, 1.0), np.zeros(36),
+                              step=1.0 / (1.3 * lmax), max_iterations=400, kkt_tolerance=1e-7, cycles=60,
+                              kkt=1e-6, lam=0.02)
+    monkeypatch.setattr(X, "solve", orig)
+    direct = X.solve(gram, d, q0, X.L1Prior(0.02, 1.0), np.zeros(36), step=1.0 / (1.01 * lmax),
+                     max_iterations=20000, gap_tolerance=1e-12)
+    assert out["stop"] == "cycles"
+    assert abs(out["last"]["objective"] - direct["last"]["objective"]) > 1e-4
