@@ -759,7 +759,7 @@ def gram_solve(dev, tol=1e-4, budget=4000, train_steps=60, tau=0.1):
 
 
 def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192, pix_arcsec=1.3, eps=1e-5,
-               cycles=10, inner_iterations=50, plain_iterations=30, lam_frac=0.02, kkt_rel=1e-4):
+               cycles=10, inner_iterations=20, plain_iterations=30, lam_frac=0.02, kkt_rel=1e-4):
     """Wide-field major-cycle imaging: four-correlation complex64 visibilities
     of 2016 baselines x `ntime` times x `nchan` channels (a 20-source sky over
     the 3 degree field; the band chosen so that |w| reaches about 6000
@@ -777,6 +777,7 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
     import torch
 
     from rang import core_weights, gram, synth
+    from rang import device as D
     from rang.measurement import WgridMeasurement
     from rang.prox_solver import L1Prior, OuterSchedule, Problem, Settings, defect_correction, solve
 
@@ -796,13 +797,20 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
         del c[k]
     torch.cuda.empty_cache()
     uvw, freq, pix = c["uvw"], c["freq"], c["pixsize"]
+    # the surrogate first, on the weights normalised as the measurement
+    # normalises them; its set-up grids through the library's shared work
+    # buffers, released before the plan is made
+    o = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+    wn = (wgt.to(torch.float64) / float(wgt.to(torch.float64).sum())).to(torch.float32).contiguous()
+    G, t_sur = clock(lambda: gram.GramOperator.new(o, uvw, freq, wn, gram.WExpansion(8, 1.0, 1e-13), eps,
+                                                  single=True))
+    del wn
+    D.release()
+    torch.cuda.empty_cache()
     m, t_prep = clock(lambda: WgridMeasurement(uvw, freq, vis, wgt, npix, pix, eps, single=True))
     del wgt
     torch.cuda.empty_cache()
-    progress(f"wide field: Stokes I {t_stokes} s, plan prepared in {t_prep} s")
-    o = [npix, npix, pix, pix, eps, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
-    G, t_sur = clock(lambda: gram.GramOperator.new(o, uvw, freq, m.wgt, gram.WExpansion(8, 1.0, 1e-13), eps,
-                                                  single=True))
+    progress(f"wide field: Stokes I {t_stokes} s, surrogate built in {t_sur} s, plan prepared in {t_prep} s")
     zeros = torch.zeros((npix, npix), dtype=torch.float32, device=dev)
     (dirty, q0), t_dirty = clock(lambda: m.residual(zeros))
     free_bytes, total_bytes = torch.cuda.mem_get_info(dev)
@@ -811,10 +819,11 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
     # the plan of a unit-weight pass (every visibility active, as here)
     plan_info = CW.plan(uvw=uvw.cpu().numpy(), freq=freq.cpu().numpy(), npix_x=npix, npix_y=npix, pixsize_x=pix,
                         pixsize_y=pix, epsilon=eps, single=True)
+    m.plan.gram(dirty)
     _, t_exact = clock(lambda: m.plan.gram(dirty))
+    G.apply(dirty)  # the first application sizes the operator's scratch
     _, t_cheap = clock(lambda: G.apply(dirty))
-    progress(f"wide field: surrogate built in {t_sur} s; one exact application {t_exact} s, one surrogate "
-             f"application {t_cheap} s")
+    progress(f"wide field: one exact application {t_exact} s, one surrogate application {t_cheap} s")
     # the step from the largest eigenvalue, by power iteration on the surrogate
     g = torch.Generator(device=dev).manual_seed(3)
     v = torch.randn((npix, npix), dtype=torch.float32, device=dev, generator=g)
@@ -843,6 +852,8 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
     progress(f"wide field: {len(dc.records) - 1} major cycles, {dc.exact_applies} exact + {dc.cheap_applies} "
              f"surrogate applications, {t_dc} s, kkt/peak {dc.last.kkt / peak:.2e}")
     reach = next((r.iteration + 1 for r in plain.records if r.kkt <= dc.last.kkt), None)
+    # exact applications the cycles needed for the residual plain FISTA ended at
+    back = next((r.outer + 1 for r in dc.records if r.kkt <= plain.last.kkt), None)
     case = {"npix": npix, "pixsize": pix, "uvw": uvw, "freq": freq, "vis": m.vis, "wgt": m.wgt, "sky": c["sky"]}
     errs = dft_errors(case, dirty, lambda img: m.plan.dirty2vis(img, weighted=True), n_pix=64)
     out = {
@@ -862,6 +873,8 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
                     "dirty": t_dirty, "exact_application": t_exact, "surrogate_application": t_cheap},
         "problem": {"lambda_over_peak": lam_frac, "kkt_over_peak_target": kkt_rel, "step": step,
                     "largest_eigenvalue": lmax},
+        "surrogate": {"w_nodes": int(G.stats()["planes"]), "stencil": 8, "theta": 1.0,
+                      "sup_error": G.stats()["sup_error"]},
         "plain_fista": {"exact_applies": plain.iterations + 1, "seconds": t_plain,
                         "kkt_over_peak": plain.last.kkt / peak, "objective": plain.last.objective,
                         "stop": plain.stop},
@@ -871,6 +884,7 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
                          "stop": dc.stop,
                          "kkt_over_peak_by_cycle": [r.kkt / peak for r in dc.records]},
         "plain_applies_to_reach_the_cycles_residual": reach,
+        "cycles_exact_applies_to_reach_the_plain_residual": back,
         "errors": errs,
         "note": "the cheap operator of the cycles is the visibility-free surrogate; the learned network is not "
                 "used here (at the Gram-solve workload its warm start changed no count)",
@@ -990,7 +1004,7 @@ def main():
                     help="run the wide-field major-cycle leg alone (no JSON contract line)")
     ap.add_argument("--wide-ntime", type=int, default=1800,
                     help="time steps of the wide-field leg (1800 is the workload)")
-    ap.add_argument("--profile", choices=("large", "small", "unet", "wstack", "ingest"),
+    ap.add_argument("--profile", choices=("large", "small", "unet", "wstack", "ingest", "gram"),
                     help="only run --warmup + --steps full vis2dirty + dirty2vis calls of one workload and print "
                          "their stage times (for profiler runs; no JSON contract line)")
     ap.add_argument("--copy-kernel", action="store_true",
@@ -1057,6 +1071,29 @@ def main():
 
     if args.only_wide:
         print(json.dumps({"wide_field": wide_field(dev, ntime=args.wide_ntime)}))
+        return
+
+    if args.profile == "gram":
+        # the visibility-free operator at the wide-field image (8192^2 at 1.3
+        # arcsec, 0.200-0.252 GHz, eight w-nodes) built from 100 time steps
+        # (its application does not depend on the visibilities)
+        from rang import gram
+
+        c = synth.imaging_case_device(100, 64, 0.200e9, 0.252e9, 8192, 1.3, dev)
+        o = [8192, 8192, c["pixsize"], c["pixsize"], 1e-5, 0, 0, 1.1, 2.6, 1, 0, 0, 0, 0, 1, -1, 0, 0]
+        wn = (c["wgt"] / c["wgt"].sum()).contiguous()
+        G = gram.GramOperator.new(o, c["uvw"], c["freq"], wn, gram.WExpansion(8, 1.0, 1e-13), 1e-5, single=True)
+        x = torch.randn((8192, 8192), dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            G.apply(x, out=y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            G.apply(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "gram", "side": 8192, "nodes": int(G.stats()["planes"]),
+                          "apply_ms": round((time.perf_counter() - t0) / max(args.steps, 1) * 1e3, 3)}))
         return
 
     if args.profile == "ingest":

@@ -516,8 +516,8 @@ void dispatch(int need, F&& f) {
     else if (need <= 8192)
         f(gic<512>{}, gic<8192>{});
     else if constexpr (sizeof(T) == 4) {
-        if (need <= 16384)
-            f(gic<512>{}, gic<16384>{});
+        if (need <= 16384)  // a line filling most of a CU's LDS: one workgroup per CU, 16 waves
+            f(gic<1024>{}, gic<16384>{});
         else
             throw GramError{"transform too long for LDS", 1};
     } else {
