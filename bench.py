@@ -31,6 +31,7 @@ skips it).
 """
 
 import argparse
+import gc
 import json
 import os
 import statistics
@@ -788,6 +789,8 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
         torch.cuda.synchronize()
         return res, round(time.perf_counter() - t, 3)
 
+    free0, _ = torch.cuda.mem_get_info(dev)
+    progress(f"wide field: {free0 / 2**30:.1f} GiB free at the start")
     c, t_gen = clock(lambda: synth.fullpol_case_device(ntime, nchan, f_lo, f_hi, npix, pix_arcsec, dev))
     ncell = c["data"].shape[0] * c["data"].shape[1]
     progress(f"wide field: {ncell} x 4 correlations formed in {t_gen} s, |w| up to {c['wmax_lambda']:.0f}")
@@ -824,6 +827,15 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
     G.apply(dirty)  # the first application sizes the operator's scratch
     _, t_cheap = clock(lambda: G.apply(dirty))
     progress(f"wide field: one exact application {t_exact} s, one surrogate application {t_cheap} s")
+    case = {"npix": npix, "pixsize": pix, "uvw": uvw, "freq": freq, "vis": m.vis, "wgt": m.wgt, "sky": c["sky"]}
+    # the data are not read again once the dirty image and the direct sum's
+    # pixels are formed (dft_errors forms those first): the test image's
+    # visibilities go into their 14 GiB instead of a new buffer
+    torch.cuda.empty_cache()
+    errs = dft_errors(case, dirty, lambda img: m.plan.dirty2vis(img, weighted=True, out=m.vis), n_pix=64)
+    del case
+    m.vis = None
+    torch.cuda.empty_cache()
     # the step from the largest eigenvalue, by power iteration on the surrogate
     g = torch.Generator(device=dev).manual_seed(3)
     v = torch.randn((npix, npix), dtype=torch.float32, device=dev, generator=g)
@@ -854,8 +866,6 @@ def wide_field(dev, ntime=1800, nchan=512, f_lo=0.200e9, f_hi=0.252e9, npix=8192
     reach = next((r.iteration + 1 for r in plain.records if r.kkt <= dc.last.kkt), None)
     # exact applications the cycles needed for the residual plain FISTA ended at
     back = next((r.outer + 1 for r in dc.records if r.kkt <= plain.last.kkt), None)
-    case = {"npix": npix, "pixsize": pix, "uvw": uvw, "freq": freq, "vis": m.vis, "wgt": m.wgt, "sky": c["sky"]}
-    errs = dft_errors(case, dirty, lambda img: m.plan.dirty2vis(img, weighted=True), n_pix=64)
     out = {
         "workload": f"2016 baselines x {ntime} times x {nchan} channels ({f_lo / 1e9:.3f}-{f_hi / 1e9:.3f} GHz), "
                     f"4 correlations (XX, XY, YX, YY) complex64, Stokes I on the GPU; {npix}x{npix} at "
@@ -1174,6 +1184,18 @@ def main():
         D.release()
         torch.cuda.empty_cache()
 
+    wide = None
+    if not args.no_wide and not args.only_small:
+        # the wide-field plan needs most of the GPU: it runs while nothing
+        # else is held, every earlier plan and kept buffer released first
+        gc.collect()
+        D.release()
+        torch.cuda.empty_cache()
+        wide = wide_field(dev, ntime=args.wide_ntime)
+        gc.collect()
+        D.release()
+        torch.cuda.empty_cache()
+
     small_line = None
     if not args.no_small:
         small = synth.config1()
@@ -1233,12 +1255,6 @@ def main():
         gram = gram_solve(dev)
         progress("Gram solve: " + json.dumps({k: gram[k].get("exact_applies") for k in
                                               ("cg", "pcg_inverse_filter", "inner_outer", "inner_outer_learned")}))
-    wide = None
-    if not args.no_wide:
-        D.release()
-        torch.cuda.empty_cache()
-        wide = wide_field(dev, ntime=args.wide_ntime)
-        torch.cuda.empty_cache()
     line = {
         "metric": METRIC,
         "value": round(value, 2),

@@ -322,13 +322,24 @@ __global__ __launch_bounds__(256) void k_spectrum(const cx<T>* __restrict__ kg, 
 }
 
 // Cells of a spectrum below `floor` in magnitude.
+// A grid-stride loop over a bounded number of blocks, the count reduced in
+// the block and added with one atomic per block (an atomic per wave on the
+// one counter serialised the launch: 38 ms for 2^28 cells).
+constexpr int kCountBlocks = 2048;
 template <class T>
 __global__ __launch_bounds__(256) void k_count_small(const T* __restrict__ spec, long long n, double floor_,
                                                      unsigned long long* __restrict__ cnt) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    unsigned long long mine = (e < n && fabs(double(spec[e])) < floor_) ? 1ull : 0ull;
+    unsigned long long mine = 0ull;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
+        mine += fabs(double(spec[e])) < floor_ ? 1ull : 0ull;
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
-    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(cnt, mine);
+    __shared__ unsigned long long wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(cnt, t);
+    }
 }
 
 // The screen factor exp(2 pi i w x(l)) of pixel (i, j): the phase reduced
@@ -851,7 +862,8 @@ GramBase* gram_build(const Options& o, const DevInputs& in, const double* freq_h
         }
         herm = std::max(herm, im_max / std::max(re_max, 1e-300));
         peak = std::max(peak, re_max);
-        k_count_small<T><<<sblocks, 256, 0, st>>>(spec, nspec, 1e-7 * re_max * scale, cnt + 1);
+        k_count_small<T><<<unsigned(std::min<long long>(kCountBlocks, (nspec + 255) / 256)), 256, 0, st>>>(
+            spec, nspec, 1e-7 * re_max * scale, cnt + 1);
         GCHK(hipGetLastError());
         unsigned long long hs = 0;
         GCHK(hipMemcpyAsync(&hs, cnt + 1, sizeof hs, hipMemcpyDeviceToHost, st));
