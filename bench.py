@@ -1191,7 +1191,13 @@ def main():
         gc.collect()
         D.release()
         torch.cuda.empty_cache()
-        wide = wide_field(dev, ntime=args.wide_ntime)
+        try:
+            wide = wide_field(dev, ntime=args.wide_ntime)
+        except (torch.OutOfMemoryError, RuntimeError, ValueError) as e:
+            # the leg needs about 270 GB of the GPU's 288 GiB; a box that
+            # cannot hold it reports why instead of losing the other legs
+            wide = {"error": f"{type(e).__name__}: {str(e)[:400]}"}
+            progress(f"wide field: not measured ({wide['error'][:120]})")
         gc.collect()
         D.release()
         torch.cuda.empty_cache()

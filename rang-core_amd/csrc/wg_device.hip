@@ -2350,6 +2350,98 @@ __global__ __launch_bounds__(256) void k_row_need(Geo g, const std::uint32_t* __
     }
 }
 
+// Items of rc records per tile in tile order (an identity tile order):
+// items[t] = ceil(count_t / rc), the last entry 0 for the exclusive scan.
+__global__ __launch_bounds__(256) void k_chunk_counts(Geo g, const std::uint32_t* __restrict__ off, std::uint32_t rc,
+                                                      std::uint32_t* __restrict__ items) {
+    const int nt = g.ntu * g.ntv;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nt) return;
+    if (t == nt) {
+        items[nt] = 0;
+        return;
+    }
+    const std::uint32_t cnt = off[(long long)(t + 1) * 4 * g.nb] - off[(long long)t * 4 * g.nb];
+    items[t] = (cnt + rc - 1) / rc;
+}
+
+// The degridding records of each item (chunk c of rc records of a tile,
+// [t0 + c rc, min(t1, t0 + (c + 1) rc))) regrouped by their cell in the tile
+// (row, then column). The degridder's lanes take neighbouring records, so
+// its staged reads then hit equal or neighbouring cells instead of cells
+// scattered over the tile (LDS bank conflicts were 57 % of its LDS cycles;
+// 61.5 -> 41.4 ms on the 464M workload). Records stay within their chunk: the
+// item's plane range (the bucket bins overlapping the chunk) is unchanged.
+// One 1024-thread workgroup per item, rc <= kSortChunk records held in
+// registers: a counting sort over the Tu Tv cells in LDS (slots within a
+// cell in atomic order).
+constexpr int kSortChunk = 4096;
+template <class T>
+__global__ __launch_bounds__(1024) void k_sort_chunks(Geo g, Rec<T>* __restrict__ recs,
+                                                      const std::uint32_t* __restrict__ off,
+                                                      const std::uint32_t* __restrict__ item_off, std::uint32_t rc) {
+    __shared__ unsigned cnt[kSortChunk];
+    __shared__ unsigned wsum[16];
+    constexpr int PER = kSortChunk / 1024;
+    int tile;
+    std::uint32_t chunk, nchunks;
+    if (!find_item(item_off, g.ntu * g.ntv, blockIdx.x, tile, chunk, nchunks)) return;
+    const std::uint32_t t0 = off[(long long)tile * 4 * g.nb], t1 = off[(long long)(tile + 1) * 4 * g.nb];
+    const std::uint32_t a = t0 + chunk * rc, b = min(t1, a + rc);
+    const int n = int(b - a);
+    const int ncell = g.Tu * g.Tv;  // <= kSortChunk, checked by the caller
+    for (int c = threadIdx.x; c < ncell; c += 1024) cnt[c] = 0u;
+    __syncthreads();
+    Rec<T> held[PER];
+    int cell[PER];
+    unsigned slot[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int sl = int(threadIdx.x) + k * 1024;
+        cell[k] = -1;
+        if (sl < n) {
+            held[k] = recs[a + sl];
+            // pos: row | column << 8 | first plane << 16, row < Tu and column < Tv
+            // (clamped: the count stays inside the table whatever the record)
+            const std::uint32_t pos = held[k].pos;
+            cell[k] = min(int(pos & 255u), g.Tu - 1) * g.Tv + min(int((pos >> 8) & 255u), g.Tv - 1);
+            slot[k] = atomicAdd(&cnt[cell[k]], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the cell counts: four per thread, then over the
+    // threads (waves, then lanes)
+    constexpr int CPT = kSortChunk / 1024;
+    unsigned v[CPT], run = 0u;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = int(threadIdx.x) * CPT + q;
+        v[q] = c < ncell ? cnt[c] : 0u;
+        run += v[q];
+    }
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    unsigned incl = run;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    unsigned base = incl - run;
+    for (int w = 0; w < wave; ++w) base += wsum[w];
+    __syncthreads();  // every count read before the bases overwrite them
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = int(threadIdx.x) * CPT + q;
+        if (c < ncell) cnt[c] = base;
+        base += v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (cell[k] >= 0) recs[a + cnt[cell[k]] + slot[k]] = held[k];
+}
+
 // d2v: degridding (wgridder.rs:2793-2901) of one item: per visibility of the
 // chunk, the S x S taps of each of its planes against the plane grid, the
 // planes added in order, then the weight and shift phase
@@ -2849,6 +2941,34 @@ constexpr std::uint32_t degrid_chunk_staged() {
 template <class T>
 constexpr std::size_t degrid_lds_max() {
     return sizeof(T) == 4 ? 150 * 1024 : 64 * 1024;
+}
+
+// Items per degridding launch and their records reordered within each item
+// (k_sort_chunks), before the degridder first reads them: the chunk size the
+// degrid pipeline will use for this geometry. RANG_DEGRID_SORT=0 leaves the
+// bucket order (a measurement knob).
+template <class T>
+void sort_degrid_chunks(Workspace& ws, const Geo& g, Rec<T>* recs, const std::uint32_t* off, hipStream_t st) {
+    static const bool on = [] {
+        const char* e = std::getenv("RANG_DEGRID_SORT");
+        return !(e && e[0] == '0');
+    }();
+    if (!on) return;
+    const int P = g.wgrid ? g.nplanes : 1;
+    const bool staged = degrid_lds<T>(g.Tu, g.Tv, g.supp, P) <= degrid_lds_max<T>();
+    const std::uint32_t rc = staged ? degrid_chunk_staged<T>() : kDegridChunk;
+    if (rc > std::uint32_t(kSortChunk) || g.Tu * g.Tv > kSortChunk) return;
+    const int nt = g.ntu * g.ntv;
+    std::uint32_t* items = static_cast<std::uint32_t*>(ws.get(kSlotItems, sizeof(std::uint32_t) * (nt + 1)));
+    std::uint32_t* item_off = static_cast<std::uint32_t*>(ws.get(kSlotItemOff, sizeof(std::uint32_t) * (nt + 1)));
+    k_chunk_counts<<<(nt + 1 + 255) / 256, 256, 0, st>>>(g, off, rc, items);
+    HIPCHK(hipGetLastError());
+    exclusive_scan(ws, items, item_off, nt + 1, st);
+    const long long nitems = read_u32(ws, item_off + nt, st);
+    if (nitems >= (1LL << 31)) throw DevError{"too many degridding items"};
+    if (nitems == 0) return;
+    k_sort_chunks<T><<<unsigned(nitems), 1024, 0, st>>>(g, recs, off, item_off, rc);
+    HIPCHK(hipGetLastError());
 }
 
 // Longest transform done in one LDS pass (fft_maxtot); longer ones take two
@@ -4232,6 +4352,7 @@ int device_dirty2vis(const Options& o, const DevInputs& in, const double* freq_h
         Rec<T>* recs;
         std::uint32_t* off;
         sort_records<T, 2, false>(ws, din, g, (long long)nvis, st, &recs, &off);
+        sort_degrid_chunks<T>(ws, g, recs, off, st);
         timer.mark("sort");
         DegridOut<T> o;
         o.vis = reinterpret_cast<cx<T>*>(vis);
@@ -4544,6 +4665,7 @@ PreparedBase* prepared_new(const Options& o, const DevInputs& in, const double* 
             unweighted.wgt = nullptr;
             unweighted.wgt_kind = kWgtNone;
             sort_records<T, 2, false>(p->ws, d, p->gd, p->nvis, st, &p->rd, &p->od, 1, &unweighted);
+            sort_degrid_chunks<T>(p->ws, p->gd, p->rd, p->od, st);
             // position of each visibility among the gridding records, through
             // a visibility-indexed map held only while the plan is made
             std::uint32_t* map = static_cast<std::uint32_t*>(p->ws.get(kSlotKeys, sizeof(std::uint32_t) * p->n));
