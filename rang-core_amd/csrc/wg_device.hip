@@ -1434,6 +1434,22 @@ __global__ __launch_bounds__(grid_threads<T>()) void k_grid_jobs(const GridJob<T
 // (gridding stage): 8 planes at 4 waves per SIMD 103.5 ms (117.4 without the
 // target), 6 at 5 waves 105.9, 4 at 6 waves 111.4.
 constexpr int kMfmaPlanes = 8;
+
+// Lane q of each row of 16 lanes, broadcast to its row (DPP row_newbcast;
+// q is a constant once the plane loop is unrolled).
+__device__ __forceinline__ int row_bcast(int v, int q) {
+    switch (q) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0x150, 0xf, 0xf, false);
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x151, 0xf, 0xf, false);
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x152, 0xf, 0xf, false);
+    case 3: return __builtin_amdgcn_mov_dpp(v, 0x153, 0xf, 0xf, false);
+    case 4: return __builtin_amdgcn_mov_dpp(v, 0x154, 0xf, 0xf, false);
+    case 5: return __builtin_amdgcn_mov_dpp(v, 0x155, 0xf, 0xf, false);
+    case 6: return __builtin_amdgcn_mov_dpp(v, 0x156, 0xf, 0xf, false);
+    default: return __builtin_amdgcn_mov_dpp(v, 0x157, 0xf, 0xf, false);
+    }
+}
+static_assert(kMfmaPlanes <= 8, "row_bcast covers planes 0-7");
 constexpr int kMfmaTile = 16;
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
 
@@ -1536,12 +1552,16 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
             // k: in [tlo, thi) on its planes, outside [0, S) (a zero lane)
             // on the item's other planes (q < q0 means tlo = 0; q past its
             // planes means thi = S or q >= np)
+            // One permute per group puts the tap of plane j at lane 16 k + j
+            // (j < kMfmaPlanes); plane q then broadcasts lane q of each row of
+            // 16 lanes (DPP row_newbcast, a register move on the VALU) instead
+            // of waiting on a permute through the LDS per plane.
             const int wbase = 4 * (16 * k + tlo - q0);
-            const int tWi = __float_as_int(tW);
+            const int wq = __builtin_amdgcn_ds_bpermute(wbase + 4 * i, __float_as_int(tW));
 #pragma unroll
             for (int q = 0; q < kMfmaPlanes; ++q) {
                 if (q < qlo || q >= qhi) continue;
-                const float kw = __int_as_float(__builtin_amdgcn_ds_bpermute(wbase + 4 * q, tWi));
+                const float kw = __int_as_float(row_bcast(wq, q));
                 const float a = A0 * kw;
                 are[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, are[q], 0, 0, 0);
                 aim[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, aim[q], 0, 0, 0);

@@ -100,3 +100,24 @@ def test_device_entry_points():
     # refusals: class out of range, non-positive super weight
     inp.class_ = 3
     assert _core.library().rang_amd_imaging_weights_dev(ctypes.byref(inp), ctypes.byref(o), st) == 1
+
+
+def test_stokes_i_on_device_tensors_matches_the_golden_case():
+    # stokes_i_device (the wide-field ingest's form, device tensors in and
+    # out) equals the reference's vectors bit for bit, flags as bool or uint8
+    import torch
+
+    from rang import core_weights as W
+
+    z = np.load(GOLD, allow_pickle=False)
+    c = golden_weights_case()
+    d = torch.device("cuda:0")
+    data = torch.from_numpy(np.ascontiguousarray(c["data"], np.complex64)).to(d)
+    wts = torch.from_numpy(np.ascontiguousarray(c["weights"], np.float32)).to(d)
+    for flags in (torch.from_numpy(np.ascontiguousarray(c["flags"], bool)).to(d),
+                  torch.from_numpy(np.ascontiguousarray(c["flags"], np.uint8)).to(d)):
+        vis, w = W.stokes_i_device(data, wts, flags)
+        assert np.array_equal(vis.cpu().numpy().view(np.uint32), z["stokes_vis"].view(np.uint32))
+        assert np.array_equal(w.cpu().numpy().view(np.uint32), z["stokes_weight"].view(np.uint32))
+    with pytest.raises(ValueError, match="four correlations"):
+        W.stokes_i_device(data[..., :3].contiguous(), wts[..., :3].contiguous(), flags[..., :3].contiguous())
