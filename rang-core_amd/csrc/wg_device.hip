@@ -1495,8 +1495,12 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
     __syncthreads();
     const std::uint32_t jbeg = chunk * rc, jend = min(s_pre[9], jbeg + rc);
     const int k = lane >> 4, i = lane & 15;
+    // lane 16 k + i holds tap i (u and w) for i < S and tap i - S (v) for
+    // S <= i < 2 S of visibility k: the u and v taps in one evaluation
+    static_assert(2 * S <= 16, "u and v taps share a row of 16 lanes");
+    const bool lane_u = i < S;
     TapReg<float, S> tr;
-    tr.load(coef, min(i, S - 1));
+    tr.load(coef, lane_u ? i : min(i - S, S - 1));
     const cx<float>* stored = static_cast<const cx<float>*>(in.stored);
     mfma_v4 are[kMfmaPlanes], aim[kMfmaPlanes];
 #pragma unroll
@@ -1529,15 +1533,15 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
             const int ou = int(pos & 255u) - 64, ov = int((pos >> 8) & 255u) - 64;
             int tlo, thi, q0, mp;
             taps_on_item(pos, p0, tlo, thi, q0, mp);
-            const float tU = tr.eval(rk.x0), tV = tr.eval(rk.y0);
+            const float tUV = tr.eval(lane_u ? rk.x0 : rk.y0);
             // w taps zero on the lanes past the support: a fetch of tap t
             // of visibility k at lane 16 k + t for t in [-8, 16) then reads
             // zero outside [0, S) (lane indices wrap modulo 64)
-            const float tW = i < S ? tr.eval(warg_of(rk)) : 0.f;
+            const float tW = lane_u ? tr.eval(warg_of(rk)) : 0.f;
             const int au = i - ou, av = i - ov;
             const bool inu = au >= 0 && au < S, inv = av >= 0 && av < S;
-            const float ku = __shfl(tU, 16 * k + (inu ? au : 0), 64);
-            const float kv = __shfl(tV, 16 * k + (inv ? av : 0), 64);
+            const float ku = __shfl(tUV, 16 * k + (inu ? au : 0), 64);
+            const float kv = __shfl(tUV, 16 * k + S + (inv ? av : 0), 64);
             const float A0 = inu ? ku : 0.f;
             const float Bk = inv ? kv : 0.f;
             const float bre = Bk * rk.re, bim = Bk * rk.im;
