@@ -1458,6 +1458,7 @@ struct MfmaShared {
     float coef[2 * ne_of<S>() * S];
     std::uint32_t start[9], pre[10];
     Rec<float> rec[64];
+    int qr[32];  // per group of 4 staged records: its planes [lo, hi) in the item
 };
 
 template <int S, bool FORM>
@@ -1467,7 +1468,8 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
                                                std::uint32_t rc, const std::uint32_t* __restrict__ item_off,
                                                const std::uint32_t* __restrict__ order,
                                                cx<float>* __restrict__ grid, std::uint32_t item, float* coef,
-                                               std::uint32_t* s_start, std::uint32_t* s_pre, Rec<float>* s_rec) {
+                                               std::uint32_t* s_start, std::uint32_t* s_pre, Rec<float>* s_rec,
+                                               int* s_qr) {
     constexpr int NE = ne_of<S>();
     const int lane = threadIdx.x;
     const int ntg = g.ntu * g.ntv * ngroups;
@@ -1524,8 +1526,23 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
             r.pos = 0u;
             r.re = r.im = 0.f;
         }
-        __syncthreads();  // the previous batch is consumed
-        s_rec[lane] = r;
+        // the planes of each group of four records: the union of their
+        // item planes, reduced over the four lanes once per batch
+        {
+            int tlo, thi, q0, mp;
+            taps_on_item(r.pos, p0, tlo, thi, q0, mp);
+            int qa = thi > tlo ? q0 : kMfmaPlanes, qb = thi > tlo ? q0 + thi - tlo : 0;
+            qa = min(qa, __shfl_xor(qa, 1, 64));
+            qb = max(qb, __shfl_xor(qb, 1, 64));
+            qa = min(qa, __shfl_xor(qa, 2, 64));
+            qb = max(qb, __shfl_xor(qb, 2, 64));
+            __syncthreads();  // the previous batch is consumed
+            s_rec[lane] = r;
+            if ((lane & 3) == 0) {
+                s_qr[lane >> 1] = qa;
+                s_qr[(lane >> 1) + 1] = min(qb, np);
+            }
+        }
         __syncthreads();
         for (int g4 = 0; g4 < nrec; g4 += 4) {
             const Rec<float>& rk = s_rec[g4 + k];
@@ -1546,12 +1563,8 @@ __device__ __forceinline__ void grid_item_mfma(Geo g, const Rec<float>* __restri
             const float Bk = inv ? kv : 0.f;
             const float bre = Bk * rk.re, bim = Bk * rk.im;
             // the group's planes: the union of its visibilities' item planes
-            const int qa = thi > tlo ? q0 : kMfmaPlanes, qb = thi > tlo ? q0 + thi - tlo : 0;
-            int qlo = min(min(__builtin_amdgcn_readlane(qa, 0), __builtin_amdgcn_readlane(qa, 16)),
-                          min(__builtin_amdgcn_readlane(qa, 32), __builtin_amdgcn_readlane(qa, 48)));
-            int qhi = max(max(__builtin_amdgcn_readlane(qb, 0), __builtin_amdgcn_readlane(qb, 16)),
-                          max(__builtin_amdgcn_readlane(qb, 32), __builtin_amdgcn_readlane(qb, 48)));
-            qhi = min(qhi, np);
+            const int qlo = __builtin_amdgcn_readfirstlane(s_qr[g4 >> 1]);
+            const int qhi = __builtin_amdgcn_readfirstlane(s_qr[(g4 >> 1) + 1]);
             // plane q of the item takes tap t = q - q0 + tlo of visibility
             // k: in [tlo, thi) on its planes, outside [0, S) (a zero lane)
             // on the item's other planes (q < q0 means tlo = 0; q past its
@@ -1606,7 +1619,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                                                   cx<float>* __restrict__ grid) {
     __shared__ MfmaShared<S> sh;
     grid_item_mfma<S, FORM>(g, recs, off, coeff, in, G, ngroups, rc, item_off, order, grid, blockIdx.x, sh.coef,
-                            sh.start, sh.pre, sh.rec);
+                            sh.start, sh.pre, sh.rec, sh.qr);
 }
 
 // Every job's items in one launch on the matrix cores (jobs as k_grid_jobs,
@@ -1623,7 +1636,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 #define MJOB_CASE(SS)                                                                                    \
     case SS:                                                                                             \
         grid_item_mfma<SS, FORM>(J.g, J.recs, J.off, J.coeff, J.in, J.G, J.ngroups, J.rc, J.item_off,    \
-                                 J.order, J.grid, item, sh.coef, sh.start, sh.pre, sh.rec);              \
+                                 J.order, J.grid, item, sh.coef, sh.start, sh.pre, sh.rec, sh.qr);       \
         break;
         MJOB_CASE(4) MJOB_CASE(5) MJOB_CASE(6) MJOB_CASE(7) MJOB_CASE(8)
 #undef MJOB_CASE
