@@ -204,23 +204,36 @@ __device__ __forceinline__ void dft_small(cx<T>* v, int sgn) {
 }
 
 // One Stockham pass of radix R over `ncol` transforms of length n in LDS.
-// BMAX bounds the butterflies per thread (ceil(ncol n / (R NT))).
-template <class T, int NT, int R, int BMAX>
+// BMAX bounds the butterflies per thread (ceil(ncol n / (R NT))). Each
+// butterfly's indices (its transform c, its position q within the output
+// stride and k within the twiddles) are formed once, in the load phase, and
+// kept for the store; with ONE (a single transform, as every row and
+// long-column kernel) c is 0 and no division by the butterfly count is made.
+// The divisions are multiply-high (quarter-rate on the VALU), so the index
+// arithmetic was as large a part of a pass as its butterflies.
+template <class T, int NT, int R, int BMAX, bool ONE>
 __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, unsigned mnb,
                                          unsigned mns, const cx<T>* tw, int sgn) {
     const int nb = n / R;
-    const int total = nb * ncol;
+    const int total = ONE ? nb : nb * ncol;
     cx<T> v[BMAX][R];
+    int cq[BMAX], cb[BMAX];  // column offset of the butterfly's transform; its output base
 #pragma unroll
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
         if (j < total) {
-            const int c = int(udiv(unsigned(j), unsigned(nb), mnb)), jj = j - c * nb;
-            const cx<T>* col = buf + c * n;
+            int c = 0, jj = j;
+            if constexpr (!ONE) {
+                c = int(udiv(unsigned(j), unsigned(nb), mnb));
+                jj = j - c * nb;
+            }
+            const int q = int(udiv(unsigned(jj), unsigned(ns), mns)), k = jj - q * ns;
+            cq[b] = c * n;
+            cb[b] = q * ns * R + k;
+            const cx<T>* col = buf + cq[b];
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = col[jj + t * nb];
             if (ns > 1) {
-                const int k = jj - int(udiv(unsigned(jj), unsigned(ns), mns)) * ns;
                 const cx<T>* w = tw + k * (R - 1);
 #pragma unroll
                 for (int t = 1; t < R; ++t) {
@@ -237,12 +250,9 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, un
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
         if (j < total) {
-            const int c = int(udiv(unsigned(j), unsigned(nb), mnb)), jj = j - c * nb;
-            const int q = int(udiv(unsigned(jj), unsigned(ns), mns)), k = jj - q * ns;
-            cx<T>* col = buf + c * n;
-            const int base = q * ns * R + k;
+            cx<T>* col = buf + cq[b];
 #pragma unroll
-            for (int t = 0; t < R; ++t) col[base + t * ns] = v[b][t];
+            for (int t = 0; t < R; ++t) col[cb[b] + t * ns] = v[b][t];
         }
     }
     __syncthreads();
@@ -250,23 +260,31 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, un
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// All passes of `d` over ncol * d.n <= MAXTOT values in LDS (synchronised
-// on entry by the caller, on exit here).
-template <class T, int NT, int MAXTOT>
-__device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
+template <class T, int NT, int MAXTOT, bool ONE>
+__device__ void fft_passes(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
     for (int s = 0; s < d.npass; ++s) {
         const cx<T>* tws = tw + d.twoff[s];
         const unsigned a = d.mnb[s], b = d.mns[s];
         switch (d.radix[s]) {
-        case 2: fft_pass<T, NT, 2, cdiv(MAXTOT, 2 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 3: fft_pass<T, NT, 3, cdiv(MAXTOT, 3 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 4: fft_pass<T, NT, 4, cdiv(MAXTOT, 4 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT)>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 2: fft_pass<T, NT, 2, cdiv(MAXTOT, 2 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 3: fft_pass<T, NT, 3, cdiv(MAXTOT, 3 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 4: fft_pass<T, NT, 4, cdiv(MAXTOT, 4 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         }
     }
+}
+
+// All passes of `d` over ncol * d.n <= MAXTOT values in LDS (synchronised
+// on entry by the caller, on exit here).
+template <class T, int NT, int MAXTOT>
+__device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
+    if (ncol == 1)
+        fft_passes<T, NT, MAXTOT, true>(buf, d, 1, tw, sgn);
+    else
+        fft_passes<T, NT, MAXTOT, false>(buf, d, ncol, tw, sgn);
 }
 
 // Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
