@@ -81,7 +81,10 @@ inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im) {
         return true;
     };
     // Odd radices first: the first pass writes at a stride of its radix, and
-    // an odd stride spreads over the LDS banks where 8 would not.
+    // an odd stride spreads over the LDS banks where 8 would not. 3 and 7
+    // together go as one radix-21 pass.
+    while (m % 21 == 0)
+        if (!push(21)) return false;
     for (int r : {3, 5, 7, 11})
         while (m % r == 0)
             if (!push(r)) return false;
@@ -191,6 +194,32 @@ __device__ __forceinline__ void dft_odd(cx<T>* v, int sgn) {
     for (int q = 0; q < R; ++q) v[q] = out[q];
 }
 
+// Radix 21 as 3 x 7 in registers by the prime-factor (Good-Thomas) mapping,
+// so no twiddles between its two stages: a[n1][n2] = v[(7 n1 + 3 n2) mod 21],
+// DFT3 along n1, DFT7 along n2, out[(7 k1 + 15 k2) mod 21] = a[k1][k2]. One
+// pass (and its two barriers) fewer wherever 3 and 7 both divide a length.
+template <class T>
+__device__ __forceinline__ void dft21(cx<T>* v, int sgn) {
+    cx<T> a[3][7];
+#pragma unroll
+    for (int n1 = 0; n1 < 3; ++n1)
+#pragma unroll
+        for (int n2 = 0; n2 < 7; ++n2) a[n1][n2] = v[(7 * n1 + 3 * n2) % 21];
+#pragma unroll
+    for (int n2 = 0; n2 < 7; ++n2) {
+        cx<T> c[3] = {a[0][n2], a[1][n2], a[2][n2]};
+        dft_odd<T, 3>(c, sgn);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) a[k1][n2] = c[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        dft_odd<T, 7>(a[k1], sgn);
+#pragma unroll
+        for (int k2 = 0; k2 < 7; ++k2) v[(7 * k1 + 15 * k2) % 21] = a[k1][k2];
+    }
+}
+
 template <class T, int R>
 __device__ __forceinline__ void dft_small(cx<T>* v, int sgn) {
     if constexpr (R == 2)
@@ -199,6 +228,8 @@ __device__ __forceinline__ void dft_small(cx<T>* v, int sgn) {
         dft4(v, sgn);
     else if constexpr (R == 8)
         dft8(v, sgn);
+    else if constexpr (R == 21)
+        dft21(v, sgn);
     else
         dft_odd<T, R>(v, sgn);
 }
@@ -272,6 +303,7 @@ __device__ void fft_passes(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* 
         case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 21: fft_pass<T, NT, 21, cdiv(MAXTOT, 21 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         }
     }
