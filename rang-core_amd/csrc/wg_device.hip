@@ -1863,7 +1863,7 @@ __global__ __launch_bounds__(NT) void k_rows_inv(const cx<T>* __restrict__ grid,
         for (int j = threadIdx.x; j < ny; j += NT) tmp[row * ny + j] = cx<T>{T(0), T(0)};
         return;
     }
-    fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, +1);
+    fft_lds<T, NT, MAXTOT, sizeof(T) == 4>(buf, d, 1, tw, +1);
     const int sh = nv - ny / 2;
     for (int j = threadIdx.x; j < ny; j += NT) {
         const int e = sh + j;
@@ -1901,7 +1901,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const cx<T>* __restrict__ tmp,
         });
     }
     __syncthreads();
-    fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, +1);
+    fft_lds<T, NT, MAXTOT, sizeof(T) == 4>(buf, d, nc, tw, +1);
     const double w0 = g.wmin + double(p) * g.dw;
     T* dst = proj + (long long)p * nx * ny;
     const int sh = nu - nx / 2;
@@ -2116,7 +2116,7 @@ __global__ __launch_bounds__(NT) void k_cols_fwd(const T* __restrict__ img,
         }
     }
     __syncthreads();
-    fft_lds<T, NT, MAXTOT>(buf, d, nc, tw, -1);
+    fft_lds<T, NT, MAXTOT, sizeof(T) == 4>(buf, d, nc, tw, -1);
     cx<T>* dst = tmp + (long long)p * nu * ny;
     if constexpr (TCM) {
         for (int e = threadIdx.x; e < nu * nc; e += NT) dst[(long long)j0 * nu + e] = buf[e];
@@ -2163,7 +2163,7 @@ __global__ __launch_bounds__(NT) void k_rows_fwd(const cx<T>* __restrict__ tmp,
         }
     }
     __syncthreads();
-    fft_lds<T, NT, MAXTOT>(buf, d, 1, tw, -1);
+    fft_lds<T, NT, MAXTOT, sizeof(T) == 4>(buf, d, 1, tw, -1);
     cx<T>* dst = grid + row * nv;
     for (int e = threadIdx.x; e < nv; e += NT) dst[e] = buf[e];
 }
@@ -2881,7 +2881,9 @@ void make_tables(Workspace& ws, const Geometry& G, const Geo& g, hipStream_t st,
     }
     ws.table_key.clear();
     std::vector<double> twu, twv;
-    if (!fft_describe(int(G.nu), tb.du, twu) || !fft_describe(int(G.nv), tb.dv, twv))
+    // single-precision plane transforms take the radix-21 pass (their kernels
+    // instantiate it; double precision keeps its registers)
+    if (!fft_describe(int(G.nu), tb.du, twu, sizeof(T) == 4) || !fft_describe(int(G.nv), tb.dv, twv, sizeof(T) == 4))
         throw DevError{"grid size is not a product of 2, 3, 5, 7 and 11"};
     tb.twu = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotTwU, twu, st, keep));
     tb.twv = reinterpret_cast<cx<T>*>(upload_real<T>(ws, kSlotTwV, twv, st, keep));

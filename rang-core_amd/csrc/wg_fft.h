@@ -70,7 +70,9 @@ __host__ __device__ __forceinline__ unsigned udiv(unsigned x, unsigned d, unsign
 }
 
 // Host: the passes of length n and its twiddle table (double precision).
-inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im) {
+// With fuse21, 3 and 7 together go as one radix-21 pass; such a plan is for
+// the transforms run by fft_lds<..., true> only.
+inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im, bool fuse21 = false) {
     d = FftDesc();
     d.n = n;
     int m = n;
@@ -81,9 +83,8 @@ inline bool fft_describe(int n, FftDesc& d, std::vector<double>& tw_re_im) {
         return true;
     };
     // Odd radices first: the first pass writes at a stride of its radix, and
-    // an odd stride spreads over the LDS banks where 8 would not. 3 and 7
-    // together go as one radix-21 pass.
-    while (m % 21 == 0)
+    // an odd stride spreads over the LDS banks where 8 would not.
+    while (fuse21 && m % 21 == 0)
         if (!push(21)) return false;
     for (int r : {3, 5, 7, 11})
         while (m % r == 0)
@@ -291,7 +292,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, un
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-template <class T, int NT, int MAXTOT, bool ONE>
+template <class T, int NT, int MAXTOT, bool ONE, bool R21>
 __device__ void fft_passes(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
     for (int s = 0; s < d.npass; ++s) {
         const cx<T>* tws = tw + d.twoff[s];
@@ -303,20 +304,24 @@ __device__ void fft_passes(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* 
         case 5: fft_pass<T, NT, 5, cdiv(MAXTOT, 5 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         case 7: fft_pass<T, NT, 7, cdiv(MAXTOT, 7 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         case 8: fft_pass<T, NT, 8, cdiv(MAXTOT, 8 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
-        case 21: fft_pass<T, NT, 21, cdiv(MAXTOT, 21 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
+        case 21:
+            if constexpr (R21) fft_pass<T, NT, 21, cdiv(MAXTOT, 21 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn);
+            break;
         default: fft_pass<T, NT, 11, cdiv(MAXTOT, 11 * NT), ONE>(buf, d.n, ncol, d.ns[s], a, b, tws, sgn); break;
         }
     }
 }
 
 // All passes of `d` over ncol * d.n <= MAXTOT values in LDS (synchronised
-// on entry by the caller, on exit here).
-template <class T, int NT, int MAXTOT>
+// on entry by the caller, on exit here). R21: the plan may hold radix-21
+// passes (fft_describe with fuse21); its code is instantiated only there, as
+// it takes registers a multi-line kernel does not have to spare.
+template <class T, int NT, int MAXTOT, bool R21 = false>
 __device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
     if (ncol == 1)
-        fft_passes<T, NT, MAXTOT, true>(buf, d, 1, tw, sgn);
+        fft_passes<T, NT, MAXTOT, true, R21>(buf, d, 1, tw, sgn);
     else
-        fft_passes<T, NT, MAXTOT, false>(buf, d, ncol, tw, sgn);
+        fft_passes<T, NT, MAXTOT, false, R21>(buf, d, ncol, tw, sgn);
 }
 
 // Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
