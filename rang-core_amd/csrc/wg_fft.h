@@ -236,20 +236,21 @@ __device__ __forceinline__ void dft_small(cx<T>* v, int sgn) {
 }
 
 // One Stockham pass of radix R over `ncol` transforms of length n in LDS.
-// BMAX bounds the butterflies per thread (ceil(ncol n / (R NT))). Each
-// butterfly's indices (its transform c, its position q within the output
-// stride and k within the twiddles) are formed once, in the load phase, and
-// kept for the store; with ONE (a single transform, as every row and
-// long-column kernel) c is 0 and no division by the butterfly count is made.
-// The divisions are multiply-high (quarter-rate on the VALU), so the index
-// arithmetic was as large a part of a pass as its butterflies.
+// BMAX bounds the butterflies per thread (ceil(ncol n / (R NT))). With ONE
+// (a single transform, as every row and long-column kernel) the butterfly's
+// line is 0, so no division by the butterfly count is made, and its output
+// base (position q within the output stride, k within the twiddles) is formed
+// once in the load phase and kept for the store: the divisions are
+// multiply-high, quarter-rate on the VALU. Several transforms recompute their
+// indices in the store phase instead, which keeps no index registers live
+// across the barrier (their kernels hold two transforms' registers).
 template <class T, int NT, int R, int BMAX, bool ONE>
 __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, unsigned mnb,
                                          unsigned mns, const cx<T>* tw, int sgn) {
     const int nb = n / R;
     const int total = ONE ? nb : nb * ncol;
     cx<T> v[BMAX][R];
-    int cq[BMAX], cb[BMAX];  // column offset of the butterfly's transform; its output base
+    int cb[ONE ? BMAX : 1];
 #pragma unroll
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
@@ -259,12 +260,17 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, un
                 c = int(udiv(unsigned(j), unsigned(nb), mnb));
                 jj = j - c * nb;
             }
-            const int q = int(udiv(unsigned(jj), unsigned(ns), mns)), k = jj - q * ns;
-            cq[b] = c * n;
-            cb[b] = q * ns * R + k;
-            const cx<T>* col = buf + cq[b];
+            const cx<T>* col = buf + c * n;
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = col[jj + t * nb];
+            int k = 0;
+            if constexpr (ONE) {
+                const int q = int(udiv(unsigned(jj), unsigned(ns), mns));
+                k = jj - q * ns;
+                cb[b] = q * ns * R + k;
+            } else if (ns > 1) {
+                k = jj - int(udiv(unsigned(jj), unsigned(ns), mns)) * ns;
+            }
             if (ns > 1) {
                 const cx<T>* w = tw + k * (R - 1);
 #pragma unroll
@@ -282,9 +288,18 @@ __device__ __forceinline__ void fft_pass(cx<T>* buf, int n, int ncol, int ns, un
     for (int b = 0; b < BMAX; ++b) {
         const int j = int(threadIdx.x) + b * NT;
         if (j < total) {
-            cx<T>* col = buf + cq[b];
+            int base;
+            cx<T>* col = buf;
+            if constexpr (ONE) {
+                base = cb[b];
+            } else {
+                const int c = int(udiv(unsigned(j), unsigned(nb), mnb)), jj = j - c * nb;
+                const int q = int(udiv(unsigned(jj), unsigned(ns), mns)), k = jj - q * ns;
+                col = buf + c * n;
+                base = q * ns * R + k;
+            }
 #pragma unroll
-            for (int t = 0; t < R; ++t) col[cb[b] + t * ns] = v[b][t];
+            for (int t = 0; t < R; ++t) col[base + t * ns] = v[b][t];
         }
     }
     __syncthreads();
@@ -313,15 +328,22 @@ __device__ void fft_passes(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* 
 }
 
 // All passes of `d` over ncol * d.n <= MAXTOT values in LDS (synchronised
-// on entry by the caller, on exit here). R21: the plan may hold radix-21
-// passes (fft_describe with fuse21); its code is instantiated only there, as
-// it takes registers a multi-line kernel does not have to spare.
-template <class T, int NT, int MAXTOT, bool R21 = false>
+// on entry by the caller, on exit here). PLANE (the exact gridder's
+// single-precision plane transforms): the plan may hold radix-21 passes
+// (fft_describe with fuse21), and a single line takes the single-line pass.
+// Other kernels keep one pass form: the extra code would take registers they
+// do not have to spare (the surrogate's column kernels, two transforms each,
+// spilled with it).
+template <class T, int NT, int MAXTOT, bool PLANE = false>
 __device__ void fft_lds(cx<T>* buf, const FftDesc& d, int ncol, const cx<T>* tw, int sgn) {
-    if (ncol == 1)
-        fft_passes<T, NT, MAXTOT, true, R21>(buf, d, 1, tw, sgn);
-    else
-        fft_passes<T, NT, MAXTOT, false, R21>(buf, d, ncol, tw, sgn);
+    if constexpr (PLANE) {
+        if (ncol == 1)
+            fft_passes<T, NT, MAXTOT, true, true>(buf, d, 1, tw, sgn);
+        else
+            fft_passes<T, NT, MAXTOT, false, true>(buf, d, ncol, tw, sgn);
+    } else {
+        fft_passes<T, NT, MAXTOT, false, false>(buf, d, ncol, tw, sgn);
+    }
 }
 
 // Plane-wise transpose of complex values, [P][R][C] -> [P][C][R], through
